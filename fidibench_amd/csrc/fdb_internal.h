@@ -1,0 +1,85 @@
+// fdb_internal.h -- declarations shared by the HIP kernels (fdb_kernels.hip)
+// and the host side of the C ABI (fdb_capi.cpp).  Not installed.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+
+#include <string>
+
+namespace fdb {
+
+struct Grid3;
+
+// Thread-local message returned by fdb_last_error().
+void set_last_error(const std::string& msg);
+
+// Number of blocks the adjoint kernel launches for these buffers (sizes
+// its per-block partial sums).
+int64_t adjoint_blocks(int lam_dtype, const Grid3& g, const void* lam_next, void* lam);
+
+// Geometry of one advect call, in the padded 3-D form every kernel uses:
+// an NDIMS < 3 problem is placed on the trailing axes and the missing
+// leading axes get extent 1 and act = 0 (no term), so the arithmetic of
+// Upwind<1> and Upwind<2> is reproduced exactly.
+struct Grid3 {
+  int64_t n[3];   // extents, axis 0 slowest, axis 2 contiguous
+  int up[3];      // upDirection: -1 if v >= 0 else +1 (upwind.cxx:34-35)
+  int act[3];     // 1 if the axis is one of the reference's NDIMS axes
+  double c[3];    // ((dt*v)*u)/dx, upwind.cxx:72
+  double dc[3];   // d c / d v = (dt*u)/dx with u fixed
+  int64_t ntot;
+};
+
+// Builds Grid3 from the reference's ctor arguments (upwind.cxx:23-49)
+// and deltaTime; returns false on invalid sizes.
+bool make_grid(int ndims, const uint64_t* n, const double* v, const double* len, double dt,
+               Grid3* g);
+
+// Bytes per stored cell for a storage dtype (FDB_F64/F32/BF16), 0 if bad.
+size_t dtype_size(int dtype);
+
+// Largest temporal-blocking depth the fused multi-step kernel supports.
+constexpr int kMaxTimeBlock = 8;
+
+// Advances `k` steps starting from bufs[*cur] (two distinct buffers of
+// the field's size); on return *cur indexes the buffer holding the
+// result and the other buffer's contents are undefined.  k == 1 is one
+// launch of the streaming kernel; k > 1 runs the temporally blocked
+// kernel when the shape fits it, else k streaming launches.  *launches
+// (optional) receives the number of kernel launches.
+hipError_t launch_advect_pass(int dtype, const Grid3& g, void* const bufs[2], int* cur, int k,
+                              hipStream_t s, int* launches);
+
+// lam = A^T lam_next; if f_t != nullptr also adds the three velocity
+// gradient dot products into acc (device, 3 x {sum, comp}).
+hipError_t launch_adjoint_step(int lam_dtype, int f_dtype, const Grid3& g, const void* lam_next,
+                               void* lam, const void* f_t, double* partials, double* acc,
+                               hipStream_t s);
+
+// Number of per-block partial slots the reductions need (each slot is
+// 6 doubles: up to three {sum, comp} pairs).
+constexpr int kReduceBlocks = 2048;
+
+// Compensated FP64 sum of n values of dtype; result {sum, comp} at out2.
+hipError_t launch_sum(int dtype, const void* x, uint64_t n, double* partials, double* out2,
+                      hipStream_t s);
+
+// lam = f - target (lam_dtype); {sum, comp} of 0.5*(f-target)^2 at out2.
+hipError_t launch_misfit(int f_dtype, const void* f, const void* target, uint64_t n,
+                         int lam_dtype, void* lam, double* partials, double* out2,
+                         hipStream_t s);
+
+// Synthetic initial field (see fdb_init_kind).  params are copied by value.
+hipError_t launch_init(int dtype, const Grid3& g, const double* len, int kind,
+                       const double* params, int nparams, uint64_t seed, void* f, hipStream_t s);
+
+// Element-wise dtype conversion on the device (n values).
+hipError_t launch_convert(int src_dtype, const void* src, int dst_dtype, void* dst, uint64_t n,
+                          hipStream_t s);
+
+// Zero the 6-double accumulator with a kernel (no memset nodes, so the
+// call stays graph-capturable).
+hipError_t launch_zero(double* p, int n, hipStream_t s);
+
+}  // namespace fdb

@@ -1,0 +1,702 @@
+// fdb_kernels.hip -- CDNA4 (gfx950) kernels of the upwind advection path.
+//
+// Arithmetic contract (pletzer/fidibench upwind/cxx/upwind.cxx:64-84): per
+// cell i and axis j = 0, 1, 2 in that order,
+//     g = g - c_j * (old[up_j(i)] - old[i]),   g starting at old[i],
+// with c_j = ((dt*v_j)*u_j)/dx_j computed once on the host in FP64.  The
+// whole file is compiled without floating-point contraction (the pragma
+// below and -ffp-contract=off), so no multiply-add is fused and the FP64
+// results equal the reference's bit for bit.
+//
+// Layout: row-major, axis 2 contiguous (upwind.cxx:39-44).  Two buffers
+// ping-pong between steps instead of the reference's per-step oldF copy.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fdb_internal.h"
+#include "../../include/fidibench_amd/fdb_upwind.h"
+
+#pragma clang fp contract(off)
+
+namespace fdb {
+namespace {
+
+// ------------------------------------------------------------ storage types
+
+__host__ __device__ inline uint16_t f32_to_bf16(float x) {
+  uint32_t b = __builtin_bit_cast(uint32_t, x);
+  if ((b & 0x7f800000u) == 0x7f800000u && (b & 0x007fffffu))
+    return (uint16_t)((b >> 16) | 0x0040u);  // quiet NaN
+  b += 0x7fffu + ((b >> 16) & 1u);           // round to nearest, ties to even
+  return (uint16_t)(b >> 16);
+}
+
+__host__ __device__ inline float bf16_to_f32(uint16_t h) {
+  return __builtin_bit_cast(float, (uint32_t)h << 16);
+}
+
+// Storage type TS and the type arithmetic is done in (M).
+template <typename TS> struct St;
+template <> struct St<double> {
+  using M = double;
+  __device__ static M ld(double x) { return x; }
+  __device__ static double st(M x) { return x; }
+};
+template <> struct St<float> {
+  using M = float;
+  __device__ static M ld(float x) { return x; }
+  __device__ static float st(M x) { return x; }
+};
+template <> struct St<uint16_t> {
+  using M = float;
+  __device__ static M ld(uint16_t x) { return bf16_to_f32(x); }
+  __device__ static uint16_t st(M x) { return f32_to_bf16(x); }
+};
+
+template <typename TS, int VEC>
+struct alignas(sizeof(TS) * VEC) Pack {
+  TS v[VEC];
+};
+
+template <typename TS, int VEC>
+__device__ inline void load_pack(const TS* p, typename St<TS>::M (&out)[VEC]) {
+  const Pack<TS, VEC> pk = *reinterpret_cast<const Pack<TS, VEC>*>(p);
+#pragma unroll
+  for (int v = 0; v < VEC; ++v) out[v] = St<TS>::ld(pk.v[v]);
+}
+
+template <typename TS, int VEC>
+__device__ inline void store_pack(TS* p, const typename St<TS>::M (&in)[VEC]) {
+  Pack<TS, VEC> pk;
+#pragma unroll
+  for (int v = 0; v < VEC; ++v) pk.v[v] = St<TS>::st(in[v]);
+  *reinterpret_cast<Pack<TS, VEC>*>(p) = pk;
+}
+
+__device__ inline int64_t wrap(int64_t x, int64_t n) {
+  return x < 0 ? x + n : (x >= n ? x - n : x);
+}
+
+// Coefficients in the arithmetic type: FP64 as computed, or rounded once
+// to float for the reduced-precision storage types.
+template <typename M> struct Coef { M c[3]; };
+
+// Everything one march kernel needs, passed by value.
+struct MarchArgs {
+  int64_t n0, n1, n2;
+  int dir0, dir1, dir2;     // neighbour offsets: +u_j forward, -u_j adjoint
+  int act0, act1, act2;
+  int64_t chunk;            // planes per block along axis 0
+  int64_t tiles2, tiles1;   // tiles along axis 2 and axis 1
+};
+
+// ------------------------------------------------------ one-step kernel --
+//
+// Block = 64 lanes along axis 2 (each lane VEC contiguous cells, one
+// 16-byte access) x WY rows along axis 1.  A block owns a tile of the
+// (axis1, axis2) plane and marches along axis 0 through `chunk` planes,
+// in the direction that makes the axis-0 neighbour the plane it loaded
+// one iteration before (kept in registers).  The axis-2 neighbour comes
+// from the adjacent lane (cross-lane shuffle) except at the tile edge,
+// the axis-1 neighbour is a load of the adjacent row (an L1/L2 hit: the
+// neighbouring wave loads that row in the same iteration).
+//
+// ADJ = true runs the transpose (directions already negated in args) and
+// also accumulates the velocity-gradient products f_t[i] * d_j(i), with
+// d_j(i) = lam_next[i + dir_j e_j] - lam_next[i], into per-block partial
+// sums (Neumaier + TwoProduct in FP64).
+template <typename TS, typename TF, int VEC, int WY, bool ADJ>
+__global__ __launch_bounds__(64 * WY) void march_kernel(const TS* __restrict__ in,
+                                                        TS* __restrict__ out, MarchArgs a,
+                                                        Coef<typename St<TS>::M> cf,
+                                                        const TF* __restrict__ ft,
+                                                        double* __restrict__ partials) {
+  using M = typename St<TS>::M;
+  const int lane = threadIdx.x;
+  const int row = threadIdx.y;
+
+  // XCD-aware tile order: consecutive hardware block ids go round-robin
+  // over the 8 XCDs; give each XCD a contiguous range of tiles so tiles
+  // that share halo rows and chunk-boundary planes share one L2.
+  int64_t b = blockIdx.x;
+  const int64_t nb = gridDim.x;
+  if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+  const int64_t ti2 = b % a.tiles2;
+  b /= a.tiles2;
+  const int64_t ti1 = b % a.tiles1;
+  const int64_t ch = b / a.tiles1;
+
+  const int64_t i2 = (ti2 * 64 + lane) * VEC;
+  const int64_t i1 = ti1 * WY + row;
+  const bool valid = (i1 < a.n1) && (i2 < a.n2);
+  const int64_t i2c = valid ? i2 : 0;  // keeps idle lanes' loads in bounds
+  const int64_t i1c = valid ? i1 : 0;
+
+  const int64_t plane = a.n1 * a.n2;
+  const int64_t off_c = i1c * a.n2 + i2c;
+  const int64_t off_1 = wrap(i1c + a.dir1, a.n1) * a.n2 + i2c;
+  const bool lo_side = a.dir2 < 0;
+  const bool edge = lo_side ? (lane == 0 || i2c == 0) : (lane == 63 || i2c + VEC >= a.n2);
+  const int64_t off_e = i1c * a.n2 + (lo_side ? wrap(i2c - 1, a.n2) : wrap(i2c + VEC, a.n2));
+
+  const int64_t p_lo = ch * a.chunk;
+  const int64_t p_hi = (p_lo + a.chunk < a.n0) ? p_lo + a.chunk : a.n0;
+  const int64_t step = a.dir0 < 0 ? 1 : -1;
+  int64_t p = a.dir0 < 0 ? p_lo : p_hi - 1;
+
+  M prev[VEC];
+  load_pack<TS, VEC>(in + wrap(p + a.dir0, a.n0) * plane + off_c, prev);
+
+  double s[3] = {0.0, 0.0, 0.0}, e[3] = {0.0, 0.0, 0.0};
+
+  for (int64_t k = p_lo; k < p_hi; ++k, p += step) {
+    const TS* base = in + p * plane;
+    M c[VEC], n1[VEC], n2[VEC];
+    load_pack<TS, VEC>(base + off_c, c);
+    load_pack<TS, VEC>(base + off_1, n1);
+    M ev = M(0);
+    if (edge) ev = St<TS>::ld(base[off_e]);
+    if (lo_side) {
+      const M sh = __shfl_up(c[VEC - 1], 1);
+      n2[0] = edge ? ev : sh;
+#pragma unroll
+      for (int v = 1; v < VEC; ++v) n2[v] = c[v - 1];
+    } else {
+      const M sh = __shfl_down(c[0], 1);
+      n2[VEC - 1] = edge ? ev : sh;
+#pragma unroll
+      for (int v = 0; v < VEC - 1; ++v) n2[v] = c[v + 1];
+    }
+    M o[VEC];
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) {
+      const M x = c[v];
+      M g = x;
+      const M d0 = prev[v] - x;
+      const M d1 = n1[v] - x;
+      const M d2 = n2[v] - x;
+      if (a.act0) g = g - cf.c[0] * d0;
+      if (a.act1) g = g - cf.c[1] * d1;
+      if (a.act2) g = g - cf.c[2] * d2;
+      o[v] = g;
+      if constexpr (ADJ) {
+        if (ft != nullptr && valid) {
+          const double fv = (double)St<TF>::ld(ft[p * plane + off_c + v]);
+          const double dd[3] = {(double)prev[v] - (double)x, (double)n1[v] - (double)x,
+                                (double)n2[v] - (double)x};
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            const double pr = fv * dd[j];
+            const double pe = __builtin_fma(fv, dd[j], -pr);  // exact product error
+            const double t = s[j] + pr;
+            e[j] += (__builtin_fabs(s[j]) >= __builtin_fabs(pr)) ? (s[j] - t) + pr : (pr - t) + s[j];
+            e[j] += pe;
+            s[j] = t;
+          }
+        }
+      }
+      prev[v] = x;
+    }
+    if (valid) store_pack<TS, VEC>(out + p * plane + off_c, o);
+  }
+
+  if constexpr (ADJ) {
+    if (ft != nullptr) {
+      // Block reduction of three {sum, comp} pairs through LDS.
+      __shared__ double red[64 * WY][6];
+      const int t = row * 64 + lane;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        red[t][2 * j] = s[j];
+        red[t][2 * j + 1] = e[j];
+      }
+      __syncthreads();
+      for (int w = 32 * WY; w > 0; w >>= 1) {
+        if (t < w) {
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            const double s1 = red[t][2 * j], s2 = red[t + w][2 * j];
+            const double sum = s1 + s2;
+            const double bb = sum - s1;
+            const double err = (s1 - (sum - bb)) + (s2 - bb);  // TwoSum
+            red[t][2 * j] = sum;
+            red[t][2 * j + 1] = red[t][2 * j + 1] + red[t + w][2 * j + 1] + err;
+          }
+        }
+        __syncthreads();
+      }
+      if (t < 6) partials[blockIdx.x * 6 + t] = red[0][t];
+    }
+  }
+}
+
+// Adds npart per-block {sum, comp} x3 partials into acc (device, 6
+// doubles) in a fixed order: one block, deterministic.
+__global__ __launch_bounds__(256) void reduce_partials_kernel(const double* __restrict__ part,
+                                                              int64_t npart, int npairs,
+                                                              double* __restrict__ acc,
+                                                              int accumulate) {
+  __shared__ double red[256][6];
+  const int t = threadIdx.x;
+  double s[3] = {0, 0, 0}, e[3] = {0, 0, 0};
+  for (int64_t i = t; i < npart; i += 256) {
+    for (int j = 0; j < npairs; ++j) {
+      const double x = part[i * 6 + 2 * j];
+      const double sum = s[j] + x;
+      const double bb = sum - s[j];
+      e[j] += (s[j] - (sum - bb)) + (x - bb) + part[i * 6 + 2 * j + 1];
+      s[j] = sum;
+    }
+  }
+  for (int j = 0; j < 3; ++j) {
+    red[t][2 * j] = s[j];
+    red[t][2 * j + 1] = e[j];
+  }
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      for (int j = 0; j < npairs; ++j) {
+        const double s1 = red[t][2 * j], s2 = red[t + w][2 * j];
+        const double sum = s1 + s2;
+        const double bb = sum - s1;
+        const double err = (s1 - (sum - bb)) + (s2 - bb);
+        red[t][2 * j] = sum;
+        red[t][2 * j + 1] = red[t][2 * j + 1] + red[t + w][2 * j + 1] + err;
+      }
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    for (int j = 0; j < npairs; ++j) {
+      double s1 = red[0][2 * j], c1 = red[0][2 * j + 1];
+      if (accumulate) {
+        const double s2 = acc[2 * j];
+        const double sum = s1 + s2;
+        const double bb = sum - s1;
+        const double err = (s1 - (sum - bb)) + (s2 - bb);
+        c1 = c1 + acc[2 * j + 1] + err;
+        s1 = sum;
+      }
+      acc[2 * j] = s1;
+      acc[2 * j + 1] = c1;
+    }
+  }
+}
+
+// ----------------------------------------------------------- reductions --
+
+template <typename TS>
+__device__ inline double widen(const TS* x, uint64_t i) {
+  return (double)St<TS>::ld(x[i]);
+}
+
+// Neumaier sum per thread, TwoSum tree per block -> partials[block*6+{0,1}].
+template <typename TS>
+__global__ __launch_bounds__(256) void sum_kernel(const TS* __restrict__ x, uint64_t n,
+                                                  double* __restrict__ partials) {
+  __shared__ double red[256][2];
+  const int t = threadIdx.x;
+  double s = 0.0, e = 0.0;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + t; i < n; i += (uint64_t)gridDim.x * 256) {
+    const double v = widen(x, i);
+    const double sum = s + v;
+    e += (__builtin_fabs(s) >= __builtin_fabs(v)) ? (s - sum) + v : (v - sum) + s;
+    s = sum;
+  }
+  red[t][0] = s;
+  red[t][1] = e;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      const double s1 = red[t][0], s2 = red[t + w][0];
+      const double sum = s1 + s2;
+      const double bb = sum - s1;
+      red[t][1] = red[t][1] + red[t + w][1] + ((s1 - (sum - bb)) + (s2 - bb));
+      red[t][0] = sum;
+    }
+    __syncthreads();
+  }
+  if (t < 2) partials[blockIdx.x * 6 + t] = red[0][t];
+}
+
+// lam = f - target; partial sums of 0.5*(f-target)^2 (product error kept).
+template <typename TF, typename TL>
+__global__ __launch_bounds__(256) void misfit_kernel(const TF* __restrict__ f,
+                                                     const TF* __restrict__ tg, uint64_t n,
+                                                     TL* __restrict__ lam,
+                                                     double* __restrict__ partials) {
+  __shared__ double red[256][2];
+  const int t = threadIdx.x;
+  double s = 0.0, e = 0.0;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + t; i < n; i += (uint64_t)gridDim.x * 256) {
+    const double d = widen(f, i) - widen(tg, i);
+    lam[i] = St<TL>::st((typename St<TL>::M)d);
+    const double h = 0.5 * d;  // exact scaling
+    const double pr = h * d;
+    const double pe = __builtin_fma(h, d, -pr);
+    const double sum = s + pr;
+    e += (__builtin_fabs(s) >= __builtin_fabs(pr)) ? (s - sum) + pr : (pr - sum) + s;
+    e += pe;
+    s = sum;
+  }
+  red[t][0] = s;
+  red[t][1] = e;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      const double s1 = red[t][0], s2 = red[t + w][0];
+      const double sum = s1 + s2;
+      const double bb = sum - s1;
+      red[t][1] = red[t][1] + red[t + w][1] + ((s1 - (sum - bb)) + (s2 - bb));
+      red[t][0] = sum;
+    }
+    __syncthreads();
+  }
+  if (t < 2) partials[blockIdx.x * 6 + t] = red[0][t];
+}
+
+__global__ void zero_kernel(double* p, int n) {
+  const int t = threadIdx.x;
+  if (t < n) p[t] = 0.0;
+}
+
+template <typename TA, typename TB>
+__global__ __launch_bounds__(256) void convert_kernel(const TA* __restrict__ a, TB* __restrict__ b,
+                                                      uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * 256) {
+    const double x = (double)St<TA>::ld(a[i]);
+    b[i] = St<TB>::st((typename St<TB>::M)x);
+  }
+}
+
+// --------------------------------------------------------- initial fields
+
+// SplitMix64 (Steele, Lea & Flood 2014), counter form: draw i of `seed`.
+__host__ __device__ inline uint64_t splitmix64(uint64_t seed, uint64_t i) {
+  uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+struct InitArgs {
+  int64_t n[3];
+  int act[3];
+  double dx[3];
+  int kind;
+  int nparams;
+  uint64_t seed;
+  double p[5 * 8 + 1];
+};
+
+template <typename TS>
+__global__ __launch_bounds__(256) void init_kernel(TS* __restrict__ f, InitArgs a) {
+  const uint64_t ntot = (uint64_t)(a.n[0] * a.n[1] * a.n[2]);
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < ntot;
+       i += (uint64_t)gridDim.x * 256) {
+    const int64_t i2 = (int64_t)(i % (uint64_t)a.n[2]);
+    const int64_t r = (int64_t)(i / (uint64_t)a.n[2]);
+    const int64_t i1 = r % a.n[1];
+    const int64_t i0 = r / a.n[1];
+    const double x[3] = {((double)i0 + 0.5) * a.dx[0], ((double)i1 + 0.5) * a.dx[1],
+                         ((double)i2 + 0.5) * a.dx[2]};
+    double val = 0.0;
+    if (a.kind == FDB_INIT_DELTA) {
+      val = (i == 0) ? 1.0 : 0.0;
+    } else if (a.kind == FDB_INIT_UNIFORM) {
+      val = a.p[0] + (a.p[1] - a.p[0]) * ((double)(splitmix64(a.seed, i) >> 11) * 0x1.0p-53);
+    } else if (a.kind == FDB_INIT_SPHERE || a.kind == FDB_INIT_GAUSS) {
+      double s = 0.0;
+      bool first = true;
+      for (int j = 0; j < 3; ++j) {
+        if (!a.act[j]) continue;
+        const double d = x[j] - a.p[j];
+        s = first ? d * d : s + d * d;
+        first = false;
+      }
+      if (a.kind == FDB_INIT_SPHERE) {
+        val = (s < a.p[3] * a.p[3]) ? 1.0 : 0.0;
+        if (a.p[4] != 0.0)
+          val = val + a.p[4] * ((double)(splitmix64(a.seed, i) >> 11) * 0x1.0p-53);
+      } else {
+        val = a.p[4] * exp(-s / a.p[3]);
+      }
+    } else if (a.kind == FDB_INIT_GAUSS_SUM) {
+      const int m = (int)a.p[0];
+      for (int k = 0; k < m; ++k) {
+        const double* q = a.p + 1 + 5 * k;
+        double s = 0.0;
+        bool first = true;
+        for (int j = 0; j < 3; ++j) {
+          if (!a.act[j]) continue;
+          const double d = x[j] - q[j];
+          s = first ? d * d : s + d * d;
+          first = false;
+        }
+        val = val + q[4] * exp(-s / (q[3] * q[3]));
+      }
+    }
+    f[i] = St<TS>::st((typename St<TS>::M)val);
+  }
+}
+
+// ------------------------------------------------------------ dispatch ---
+
+template <typename TS>
+constexpr int max_vec() { return (int)(16 / sizeof(TS)); }
+
+template <typename TS, typename TF, bool ADJ>
+hipError_t march_dispatch(const Grid3& g, const TS* in, TS* out, const int dirs[3],
+                          const TF* ft, double* partials, int64_t* nblocks_out,
+                          hipStream_t s, bool dry) {
+  using M = typename St<TS>::M;
+  constexpr int WY = 4;
+  MarchArgs a;
+  a.n0 = g.n[0];
+  a.n1 = g.n[1];
+  a.n2 = g.n[2];
+  a.dir0 = dirs[0];
+  a.dir1 = dirs[1];
+  a.dir2 = dirs[2];
+  a.act0 = g.act[0];
+  a.act1 = g.act[1];
+  a.act2 = g.act[2];
+  Coef<M> cf;
+  for (int j = 0; j < 3; ++j) cf.c[j] = (M)g.c[j];
+
+  // Widest access that keeps every row start aligned.
+  int vec = max_vec<TS>();
+  const uintptr_t align_bits = (uintptr_t)in | (uintptr_t)out;
+  while (vec > 1 && ((a.n2 % vec) != 0 || (align_bits % (vec * sizeof(TS))) != 0)) vec >>= 1;
+  a.tiles2 = (a.n2 + 64 * vec - 1) / (64 * vec);
+  a.tiles1 = (a.n1 + WY - 1) / WY;
+  // Planes per block: enough blocks to fill 256 CUs several times over.
+  int64_t tiles = a.tiles2 * a.tiles1;
+  int64_t chunk = 16;
+  while (chunk > 1 && tiles * ((a.n0 + chunk - 1) / chunk) < 2048) chunk >>= 1;
+  a.chunk = chunk;
+  const int64_t nchunks = (a.n0 + chunk - 1) / chunk;
+  const int64_t nb = tiles * nchunks;
+  if (nblocks_out) *nblocks_out = nb;
+  if (dry) return hipSuccess;
+  if (nb <= 0 || nb > 0x7fffffffLL) return hipErrorInvalidConfiguration;
+  const dim3 block(64, WY);
+  switch (vec) {
+#define FDB_LAUNCH(V)                                                                        \
+  case V:                                                                                    \
+    if constexpr (V <= max_vec<TS>()) {                                                      \
+      hipLaunchKernelGGL((march_kernel<TS, TF, V, WY, ADJ>), dim3((unsigned)nb), block, 0, s, \
+                         in, out, a, cf, ft, partials);                                      \
+    }                                                                                        \
+    break;
+    FDB_LAUNCH(8)
+    FDB_LAUNCH(4)
+    FDB_LAUNCH(2)
+    FDB_LAUNCH(1)
+#undef FDB_LAUNCH
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <typename TS>
+hipError_t step_typed(const Grid3& g, const void* in, void* out, hipStream_t s) {
+  return march_dispatch<TS, TS, false>(g, (const TS*)in, (TS*)out, g.up, (const TS*)nullptr,
+                                       nullptr, nullptr, s, false);
+}
+
+}  // namespace
+
+// --------------------------------------------------------------- public --
+
+size_t dtype_size(int dtype) {
+  switch (dtype) {
+    case FDB_F64: return 8;
+    case FDB_F32: return 4;
+    case FDB_BF16: return 2;
+    default: return 0;
+  }
+}
+
+bool make_grid(int ndims, const uint64_t* n, const double* v, const double* len, double dt,
+               Grid3* g) {
+  if (ndims < 1 || ndims > 3 || !n || !v || !len) return false;
+  const int off = 3 - ndims;
+  g->ntot = 1;
+  for (int j = 0; j < 3; ++j) {
+    g->n[j] = 1;
+    g->up[j] = -1;
+    g->act[j] = 0;
+    g->c[j] = 0.0;
+    g->dc[j] = 0.0;
+  }
+  for (int r = 0; r < ndims; ++r) {
+    const int j = r + off;
+    if (n[r] == 0 || n[r] > (uint64_t)INT64_MAX) return false;
+    g->n[j] = (int64_t)n[r];
+    g->act[j] = 1;
+    g->up[j] = (v[r] < 0.) ? +1 : -1;               // upwind.cxx:34-35
+    const double dx = len[r] / (double)n[r];         // upwind.cxx:36
+    g->c[j] = dt * v[r] * (double)g->up[j] / dx;     // upwind.cxx:72
+    g->dc[j] = dt * (double)g->up[j] / dx;
+    if (g->ntot > INT64_MAX / g->n[j]) return false;
+    g->ntot *= g->n[j];
+  }
+  return true;
+}
+
+hipError_t launch_advect_pass(int dtype, const Grid3& g, void* const bufs[2], int* cur, int k,
+                              hipStream_t s, int* launches) {
+  if (bufs[0] == bufs[1] || (*cur != 0 && *cur != 1)) return hipErrorInvalidValue;
+  hipError_t err = hipSuccess;
+  for (int i = 0; i < k; ++i) {
+    const void* src = bufs[*cur];
+    void* dst = bufs[1 - *cur];
+    switch (dtype) {
+      case FDB_F64: err = step_typed<double>(g, src, dst, s); break;
+      case FDB_F32: err = step_typed<float>(g, src, dst, s); break;
+      case FDB_BF16: err = step_typed<uint16_t>(g, src, dst, s); break;
+      default: return hipErrorInvalidValue;
+    }
+    if (err != hipSuccess) return err;
+    *cur = 1 - *cur;
+  }
+  if (launches) *launches = k;
+  return err;
+}
+
+hipError_t launch_adjoint_step(int lam_dtype, int f_dtype, const Grid3& g, const void* lam_next,
+                               void* lam, const void* f_t, double* partials, double* acc,
+                               hipStream_t s) {
+  const int dirs[3] = {-g.up[0], -g.up[1], -g.up[2]};
+  int64_t nb = 0;
+  hipError_t err;
+#define FDB_ADJ(TL, TF)                                                                      \
+  err = march_dispatch<TL, TF, true>(g, (const TL*)lam_next, (TL*)lam, dirs, (const TF*)f_t, \
+                                     partials, &nb, s, false)
+  if (lam_dtype == FDB_F64 && f_dtype == FDB_F64) FDB_ADJ(double, double);
+  else if (lam_dtype == FDB_F64 && f_dtype == FDB_F32) FDB_ADJ(double, float);
+  else if (lam_dtype == FDB_F64 && f_dtype == FDB_BF16) FDB_ADJ(double, uint16_t);
+  else if (lam_dtype == FDB_F32 && f_dtype == FDB_F64) FDB_ADJ(float, double);
+  else if (lam_dtype == FDB_F32 && f_dtype == FDB_F32) FDB_ADJ(float, float);
+  else if (lam_dtype == FDB_F32 && f_dtype == FDB_BF16) FDB_ADJ(float, uint16_t);
+  else return hipErrorInvalidValue;
+#undef FDB_ADJ
+  if (err != hipSuccess || f_t == nullptr) return err;
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, partials, nb, 3, acc, 1);
+  return hipGetLastError();
+}
+
+int64_t adjoint_blocks(int lam_dtype, const Grid3& g, const void* lam_next, void* lam) {
+  const int dirs[3] = {-g.up[0], -g.up[1], -g.up[2]};
+  int64_t nb = 0;
+  if (lam_dtype == FDB_F64)
+    (void)march_dispatch<double, double, true>(g, (const double*)lam_next, (double*)lam, dirs,
+                                               nullptr, nullptr, &nb, nullptr, true);
+  else if (lam_dtype == FDB_F32)
+    (void)march_dispatch<float, float, true>(g, (const float*)lam_next, (float*)lam, dirs,
+                                             nullptr, nullptr, &nb, nullptr, true);
+  return nb;
+}
+
+hipError_t launch_sum(int dtype, const void* x, uint64_t n, double* partials, double* out2,
+                      hipStream_t s) {
+  const dim3 grid(kReduceBlocks), block(256);
+  switch (dtype) {
+    case FDB_F64: hipLaunchKernelGGL(sum_kernel<double>, grid, block, 0, s, (const double*)x, n, partials); break;
+    case FDB_F32: hipLaunchKernelGGL(sum_kernel<float>, grid, block, 0, s, (const float*)x, n, partials); break;
+    case FDB_BF16: hipLaunchKernelGGL(sum_kernel<uint16_t>, grid, block, 0, s, (const uint16_t*)x, n, partials); break;
+    default: return hipErrorInvalidValue;
+  }
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, partials,
+                     (int64_t)kReduceBlocks, 1, out2, 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_misfit(int f_dtype, const void* f, const void* target, uint64_t n,
+                         int lam_dtype, void* lam, double* partials, double* out2,
+                         hipStream_t s) {
+  const dim3 grid(kReduceBlocks), block(256);
+#define FDB_MF(TF, TL)                                                                      \
+  hipLaunchKernelGGL((misfit_kernel<TF, TL>), grid, block, 0, s, (const TF*)f,              \
+                     (const TF*)target, n, (TL*)lam, partials)
+  if (f_dtype == FDB_F64 && lam_dtype == FDB_F64) FDB_MF(double, double);
+  else if (f_dtype == FDB_F32 && lam_dtype == FDB_F64) FDB_MF(float, double);
+  else if (f_dtype == FDB_BF16 && lam_dtype == FDB_F64) FDB_MF(uint16_t, double);
+  else if (f_dtype == FDB_F64 && lam_dtype == FDB_F32) FDB_MF(double, float);
+  else if (f_dtype == FDB_F32 && lam_dtype == FDB_F32) FDB_MF(float, float);
+  else if (f_dtype == FDB_BF16 && lam_dtype == FDB_F32) FDB_MF(uint16_t, float);
+  else return hipErrorInvalidValue;
+#undef FDB_MF
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, partials,
+                     (int64_t)kReduceBlocks, 1, out2, 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_init(int dtype, const Grid3& g, const double* len, int kind,
+                       const double* params, int nparams, uint64_t seed, void* f,
+                       hipStream_t s) {
+  InitArgs a = {};
+  int r = 0;
+  for (int j = 0; j < 3; ++j) {
+    a.n[j] = g.n[j];
+    a.act[j] = g.act[j];
+    a.dx[j] = g.act[j] ? len[r++] / (double)g.n[j] : 1.0;
+  }
+  a.kind = kind;
+  a.seed = seed;
+  const int cap = (int)(sizeof(a.p) / sizeof(a.p[0]));
+  if (nparams < 0 || nparams > cap) return hipErrorInvalidValue;
+  a.nparams = nparams;
+  for (int i = 0; i < nparams; ++i) a.p[i] = params[i];
+  // Centres are given per user axis (first ndims entries of each triple);
+  // move them onto the padded trailing axes the kernel indexes.
+  int nd = g.act[0] + g.act[1] + g.act[2];
+  auto remap = [&](double* c3) {
+    double u[3] = {c3[0], c3[1], c3[2]};
+    for (int j = 0; j < 3; ++j) c3[j] = (j >= 3 - nd) ? u[j - (3 - nd)] : 0.0;
+  };
+  if (kind == FDB_INIT_SPHERE || kind == FDB_INIT_GAUSS) {
+    if (nparams < 5) return hipErrorInvalidValue;
+    remap(a.p);
+  } else if (kind == FDB_INIT_GAUSS_SUM) {
+    if (nparams < 1 || (int)a.p[0] < 0 || 1 + 5 * (int)a.p[0] > nparams) return hipErrorInvalidValue;
+    for (int k = 0; k < (int)a.p[0]; ++k) remap(a.p + 1 + 5 * k);
+  } else if (kind == FDB_INIT_UNIFORM) {
+    if (nparams < 2) return hipErrorInvalidValue;
+  } else if (kind != FDB_INIT_DELTA) {
+    return hipErrorInvalidValue;
+  }
+  const dim3 grid(8192), block(256);
+  switch (dtype) {
+    case FDB_F64: hipLaunchKernelGGL(init_kernel<double>, grid, block, 0, s, (double*)f, a); break;
+    case FDB_F32: hipLaunchKernelGGL(init_kernel<float>, grid, block, 0, s, (float*)f, a); break;
+    case FDB_BF16: hipLaunchKernelGGL(init_kernel<uint16_t>, grid, block, 0, s, (uint16_t*)f, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_convert(int sd, const void* src, int dd, void* dst, uint64_t n, hipStream_t s) {
+  const dim3 grid(4096), block(256);
+#define FDB_CV(TA, TB) \
+  hipLaunchKernelGGL((convert_kernel<TA, TB>), grid, block, 0, s, (const TA*)src, (TB*)dst, n)
+  if (sd == FDB_F64 && dd == FDB_F32) FDB_CV(double, float);
+  else if (sd == FDB_F64 && dd == FDB_BF16) FDB_CV(double, uint16_t);
+  else if (sd == FDB_F32 && dd == FDB_F64) FDB_CV(float, double);
+  else if (sd == FDB_F32 && dd == FDB_BF16) FDB_CV(float, uint16_t);
+  else if (sd == FDB_BF16 && dd == FDB_F64) FDB_CV(uint16_t, double);
+  else if (sd == FDB_BF16 && dd == FDB_F32) FDB_CV(uint16_t, float);
+  else return hipErrorInvalidValue;
+#undef FDB_CV
+  return hipGetLastError();
+}
+
+hipError_t launch_zero(double* p, int n, hipStream_t s) {
+  hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, p, n);
+  return hipGetLastError();
+}
+
+}  // namespace fdb

@@ -1,0 +1,379 @@
+// fdb_tape.cpp -- reverse-mode gradient of a loss of f_T with respect to
+// the initial field and the constant velocity, with binomial
+// checkpointing.  No counterpart in the reference (an extension of the
+// Upwind path); the adjoint kernel itself is in fdb_kernels.hip.
+//
+// Schedule: reversing n steps with c free snapshot slots (the initial
+// state is held separately) is the classical binomial problem with
+// s = c + 1 snapshots (Griewank 1992; Griewank & Walther, ACM TOMS 26,
+// 2000).  With the repetition number r = min{r : C(s+r, s) >= n}, the
+// first checkpoint is placed at distance
+//     m = max(1, C(s+r-2, s), n - C(s-1+r, s-1))
+// and the two parts are reversed recursively; the total number of forward
+// steps is then r*n - C(s+r, s+1), the optimum.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/fidibench_amd/fdb_upwind.h"
+#include "fdb_internal.h"
+
+namespace {
+
+// C(s + r, s), saturating well below INT64_MAX.
+int64_t beta(int64_t s, int64_t r) {
+  if (r < 0) return 0;
+  const int64_t cap = (int64_t)1 << 60;
+  int64_t b = 1;
+  for (int64_t i = 1; i <= std::min(s, r); ++i) {
+    // b = C(s+r, i) computed incrementally; exact while below the cap.
+    const __int128 nb = (__int128)b * (s + r - i + 1) / i;
+    if (nb > cap) return cap;
+    b = (int64_t)nb;
+  }
+  return b;
+}
+
+int64_t repetitions(int64_t n, int64_t s) {
+  int64_t r = 0;
+  while (beta(s, r) < n) ++r;
+  return r;
+}
+
+struct Plan {
+  std::vector<std::pair<int, int>> acts;
+  int64_t forward = 0;
+};
+
+// Precondition: the working state is state `s` and the top stored state
+// is state `s`.  Emits the actions that run the adjoint steps e-1 .. s.
+void treeverse(Plan& p, int s, int e, int c) {
+  const int n = e - s;
+  if (n == 1) {
+    p.acts.push_back({FDB_CK_ADJOINT, s});
+    return;
+  }
+  if (c == 0) {
+    for (int t = e - 1; t >= s; --t) {
+      p.acts.push_back({FDB_CK_RESTORE, 0});
+      if (t > s) {
+        p.acts.push_back({FDB_CK_ADVANCE, t - s});
+        p.forward += t - s;
+      }
+      p.acts.push_back({FDB_CK_ADJOINT, t});
+    }
+    return;
+  }
+  const int64_t S = c + 1;
+  const int64_t r = repetitions(n, S);
+  int64_t m = std::max<int64_t>({1, beta(S, r - 2), (int64_t)n - beta(S - 1, r)});
+  m = std::min<int64_t>(m, n - 1);
+  p.acts.push_back({FDB_CK_ADVANCE, (int)m});
+  p.forward += m;
+  p.acts.push_back({FDB_CK_STORE, s + (int)m});
+  treeverse(p, s + (int)m, e, c - 1);
+  p.acts.push_back({FDB_CK_POP, 0});
+  p.acts.push_back({FDB_CK_RESTORE, 0});
+  treeverse(p, s, s + (int)m, c);
+}
+
+Plan make_plan(int nsteps, int snaps) {
+  Plan p;
+  if (nsteps > 0) treeverse(p, 0, nsteps, std::max(0, std::min(snaps, nsteps - 1)));
+  return p;
+}
+
+int tfail(int code, const std::string& m) {
+  fdb::set_last_error(m);
+  return code;
+}
+
+}  // namespace
+
+struct fdb_tape {
+  int ndims = 0;
+  uint64_t n[3] = {1, 1, 1};
+  double v[3] = {0, 0, 0}, len[3] = {0, 0, 0};
+  double dt = 0.0;
+  int nsteps = 0;
+  int f_dtype = FDB_F64, lam_dtype = FDB_F64;
+  fdb::Grid3 g{};
+  size_t fbytes = 0, lbytes = 0;
+  int snaps = 0;
+  Plan plan;
+  size_t resume = 0;  // first plan action left for backward
+  // device memory
+  void* base = nullptr;            // state 0
+  std::vector<void*> slots;        // snapshot stack storage
+  std::vector<int> stack_steps;    // steps of the stored states (top = back)
+  void* work[2] = {nullptr, nullptr};
+  int wcur = 0;
+  int wstep = -1;                  // state held in work[wcur]
+  int time_block = fdb::kMaxTimeBlock;  // steps per forward pass
+  void* lam_tmp = nullptr;
+  double* partials = nullptr;
+  size_t partials_bytes = 0;
+  double* acc = nullptr;
+  bool forward_done = false;
+  int64_t fwd_steps = 0, adj_steps = 0;
+};
+
+namespace {
+
+#define TAPE_HIP(call)                                                          \
+  do {                                                                          \
+    hipError_t _e = (call);                                                     \
+    if (_e != hipSuccess) {                                                     \
+      (void)hipGetLastError();                                                  \
+      return tfail(_e == hipErrorOutOfMemory ? FDB_ERR_NOMEM : FDB_ERR_HIP,     \
+                   std::string(#call) + ": " + hipGetErrorString(_e));          \
+    }                                                                           \
+  } while (0)
+
+void tape_free(fdb_tape* t) {
+  if (!t) return;
+  (void)hipDeviceSynchronize();
+  if (t->base) (void)hipFree(t->base);
+  for (void* s : t->slots) (void)hipFree(s);
+  for (void* w : t->work)
+    if (w) (void)hipFree(w);
+  if (t->lam_tmp) (void)hipFree(t->lam_tmp);
+  if (t->partials) (void)hipFree(t->partials);
+  if (t->acc) (void)hipFree(t->acc);
+  delete t;
+}
+
+// Advance the working state by a steps.
+int advance(fdb_tape* t, int a, hipStream_t s) {
+  int left = a;
+  while (left > 0) {
+    const int k = std::min(left, t->time_block);
+    TAPE_HIP(fdb::launch_advect_pass(t->f_dtype, t->g, t->work, &t->wcur, k, s, nullptr));
+    left -= k;
+  }
+  t->wstep += a;
+  t->fwd_steps += a;
+  return FDB_OK;
+}
+
+void* top_buffer(fdb_tape* t) {
+  return t->stack_steps.size() == 1 ? t->base : t->slots[t->stack_steps.size() - 2];
+}
+
+}  // namespace
+
+extern "C" {
+
+int fdb_ckpt_plan(int nsteps, int snaps, int* actions, int* args, int cap,
+                  int64_t* forward_steps) {
+  if (nsteps < 0 || snaps < 0 || cap < 0) return tfail(FDB_ERR_INVALID, "ckpt_plan: negative");
+  Plan p = make_plan(nsteps, snaps);
+  for (size_t i = 0; i < p.acts.size() && (int)i < cap; ++i) {
+    if (actions) actions[i] = p.acts[i].first;
+    if (args) args[i] = p.acts[i].second;
+  }
+  if (forward_steps) *forward_steps = p.forward;
+  return (int)std::min<size_t>(p.acts.size(), (size_t)INT32_MAX);
+}
+
+int fdb_tape_create(int ndims, const uint64_t* num_cells, const double* velocity,
+                    const double* lengths, double delta_time, int nsteps, int f_dtype,
+                    int lam_dtype, uint64_t max_bytes, fdb_tape_t** out) {
+  if (!out) return tfail(FDB_ERR_INVALID, "tape_create: out is null");
+  *out = nullptr;
+  fdb::Grid3 g;
+  if (!fdb::make_grid(ndims, num_cells, velocity, lengths, delta_time, &g))
+    return tfail(FDB_ERR_INVALID, "tape_create: invalid geometry");
+  if (nsteps < 1) return tfail(FDB_ERR_INVALID, "tape_create: nsteps must be >= 1");
+  if (fdb::dtype_size(f_dtype) == 0) return tfail(FDB_ERR_INVALID, "tape_create: f dtype");
+  if (lam_dtype != FDB_F64 && lam_dtype != FDB_F32)
+    return tfail(FDB_ERR_INVALID, "tape_create: lam dtype must be F64 or F32");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return tfail(FDB_ERR_NODEV, "no HIP device available");
+  fdb_tape* t = new fdb_tape;
+  t->ndims = ndims;
+  for (int j = 0; j < ndims; ++j) {
+    t->n[j] = num_cells[j];
+    t->v[j] = velocity[j];
+    t->len[j] = lengths[j];
+  }
+  t->dt = delta_time;
+  t->nsteps = nsteps;
+  t->f_dtype = f_dtype;
+  t->lam_dtype = lam_dtype;
+  t->g = g;
+  t->fbytes = (size_t)g.ntot * fdb::dtype_size(f_dtype);
+  t->lbytes = (size_t)g.ntot * fdb::dtype_size(lam_dtype);
+  // Fixed: base + two working buffers + one adjoint buffer.
+  const uint64_t fixed = 3 * (uint64_t)t->fbytes + t->lbytes;
+  if (max_bytes < fixed) {
+    tape_free(t);
+    return tfail(FDB_ERR_NOMEM, "tape_create: max_bytes below the fixed working set (" +
+                                    std::to_string(fixed) + " bytes)");
+  }
+  t->snaps = (int)std::min<uint64_t>((max_bytes - fixed) / t->fbytes, (uint64_t)(nsteps - 1));
+  t->plan = make_plan(nsteps, t->snaps);
+  auto fail_free = [&](hipError_t e, const char* what) {
+    (void)hipGetLastError();
+    tape_free(t);
+    return tfail(e == hipErrorOutOfMemory ? FDB_ERR_NOMEM : FDB_ERR_HIP,
+                 std::string(what) + ": " + hipGetErrorString(e));
+  };
+  hipError_t e;
+  if ((e = hipMalloc(&t->base, t->fbytes)) != hipSuccess) return fail_free(e, "hipMalloc base");
+  for (void*& w : t->work)
+    if ((e = hipMalloc(&w, t->fbytes)) != hipSuccess) return fail_free(e, "hipMalloc work");
+  if ((e = hipMalloc(&t->lam_tmp, t->lbytes)) != hipSuccess) return fail_free(e, "hipMalloc lam");
+  t->slots.resize(t->snaps, nullptr);
+  for (void*& s : t->slots)
+    if ((e = hipMalloc(&s, t->fbytes)) != hipSuccess) return fail_free(e, "hipMalloc snapshot");
+  if ((e = hipMalloc((void**)&t->acc, 6 * sizeof(double))) != hipSuccess)
+    return fail_free(e, "hipMalloc acc");
+  *out = t;
+  return FDB_OK;
+}
+
+int fdb_tape_forward(fdb_tape_t* t, const void* f0, void* fT, void* stream) {
+  if (!t || !f0 || !fT) return tfail(FDB_ERR_INVALID, "tape_forward: null argument");
+  hipStream_t s = (hipStream_t)stream;
+  t->fwd_steps = 0;
+  t->adj_steps = 0;
+  TAPE_HIP(hipMemcpyAsync(t->base, f0, t->fbytes, hipMemcpyDeviceToDevice, s));
+  TAPE_HIP(hipMemcpyAsync(t->work[0], f0, t->fbytes, hipMemcpyDeviceToDevice, s));
+  t->wcur = 0;
+  t->wstep = 0;
+  t->stack_steps.assign(1, 0);
+  // The first descent of the plan: advances and stores, up to the first
+  // restore or adjoint action.
+  size_t i = 0;
+  for (; i < t->plan.acts.size(); ++i) {
+    const auto& a = t->plan.acts[i];
+    if (a.first == FDB_CK_ADVANCE) {
+      if (int rc = advance(t, a.second, s)) return rc;
+    } else if (a.first == FDB_CK_STORE) {
+      TAPE_HIP(hipMemcpyAsync(t->slots[t->stack_steps.size() - 1], t->work[t->wcur], t->fbytes,
+                              hipMemcpyDeviceToDevice, s));
+      t->stack_steps.push_back(t->wstep);
+    } else {
+      break;
+    }
+  }
+  t->resume = i;
+  // Continue to state nsteps-1 in the working buffers (the backward's
+  // first adjoint needs it), then one step into fT.
+  if (int rc = advance(t, t->nsteps - 1 - t->wstep, s)) return rc;
+  void* const last[2] = {t->work[t->wcur], fT};
+  int lc = 0;
+  TAPE_HIP(fdb::launch_advect_pass(t->f_dtype, t->g, last, &lc, 1, s, nullptr));
+  t->fwd_steps += 1;
+  TAPE_HIP(hipStreamSynchronize(s));
+  t->forward_done = true;
+  return FDB_OK;
+}
+
+int fdb_tape_backward(fdb_tape_t* t, const void* lam_T, void* lam0, double* grad_v,
+                      void* stream) {
+  if (!t || !lam_T || !lam0 || !grad_v)
+    return tfail(FDB_ERR_INVALID, "tape_backward: null argument");
+  if (lam_T == lam0) return tfail(FDB_ERR_INVALID, "tape_backward: lam_T and lam0 must differ");
+  if (!t->forward_done) return tfail(FDB_ERR_INVALID, "tape_backward: run tape_forward first");
+  hipStream_t s = (hipStream_t)stream;
+  // Partial-sum scratch for the adjoint kernel, sized for every buffer
+  // pairing the sweep uses.
+  int64_t nb = 0;
+  nb = std::max(nb, fdb::adjoint_blocks(t->lam_dtype, t->g, lam_T, lam0));
+  nb = std::max(nb, fdb::adjoint_blocks(t->lam_dtype, t->g, lam_T, t->lam_tmp));
+  nb = std::max(nb, fdb::adjoint_blocks(t->lam_dtype, t->g, t->lam_tmp, lam0));
+  nb = std::max(nb, fdb::adjoint_blocks(t->lam_dtype, t->g, lam0, t->lam_tmp));
+  const size_t pb = (size_t)nb * 6 * sizeof(double);
+  if (pb > t->partials_bytes) {
+    if (t->partials) TAPE_HIP(hipFree(t->partials));
+    t->partials = nullptr;
+    t->partials_bytes = 0;
+    TAPE_HIP(hipMalloc((void**)&t->partials, pb));
+    t->partials_bytes = pb;
+  }
+  TAPE_HIP(fdb::launch_zero(t->acc, 6, s));
+  const void* lam_next = lam_T;
+  int done = 0;  // adjoint steps taken
+  const auto& acts = t->plan.acts;
+  for (size_t i = t->resume; i < acts.size(); ++i) {
+    const auto& a = acts[i];
+    switch (a.first) {
+      case FDB_CK_ADVANCE:
+        if (int rc = advance(t, a.second, s)) return rc;
+        break;
+      case FDB_CK_STORE:
+        TAPE_HIP(hipMemcpyAsync(t->slots[t->stack_steps.size() - 1], t->work[t->wcur],
+                                t->fbytes, hipMemcpyDeviceToDevice, s));
+        t->stack_steps.push_back(t->wstep);
+        break;
+      case FDB_CK_POP:
+        t->stack_steps.pop_back();
+        break;
+      case FDB_CK_RESTORE: {
+        // Skip the copy (and the advance after it) when the working
+        // buffer already holds the state they would produce.
+        const int top = t->stack_steps.back();
+        const bool adv = i + 1 < acts.size() && acts[i + 1].first == FDB_CK_ADVANCE;
+        const int target = top + (adv ? acts[i + 1].second : 0);
+        if (t->wstep == target) {
+          if (adv) ++i;
+          break;
+        }
+        TAPE_HIP(hipMemcpyAsync(t->work[t->wcur], top_buffer(t), t->fbytes,
+                                hipMemcpyDeviceToDevice, s));
+        t->wstep = top;
+        break;
+      }
+      case FDB_CK_ADJOINT: {
+        if (t->wstep != a.second)
+          return tfail(FDB_ERR_INVALID, "tape_backward: schedule out of step (internal)");
+        ++done;
+        // The last adjoint step writes into lam0.
+        void* dst = ((t->nsteps - done) % 2 == 0) ? lam0 : t->lam_tmp;
+        TAPE_HIP(fdb::launch_adjoint_step(t->lam_dtype, t->f_dtype, t->g, lam_next, dst,
+                                          t->work[t->wcur], t->partials, t->acc, s));
+        lam_next = dst;
+        ++t->adj_steps;
+        break;
+      }
+      default:
+        return tfail(FDB_ERR_INVALID, "tape_backward: bad action (internal)");
+    }
+  }
+  if (done != t->nsteps) return tfail(FDB_ERR_INVALID, "tape_backward: incomplete (internal)");
+  double acc[6];
+  TAPE_HIP(hipMemcpyAsync(acc, t->acc, sizeof(acc), hipMemcpyDeviceToHost, s));
+  TAPE_HIP(hipStreamSynchronize(s));
+  const int off = 3 - t->ndims;
+  for (int r = 0; r < t->ndims; ++r) {
+    const int j = r + off;
+    grad_v[r] = -t->g.dc[j] * (acc[2 * j] + acc[2 * j + 1]);
+  }
+  // The sweep consumed the stored states: another backward needs another
+  // forward.
+  t->forward_done = false;
+  return FDB_OK;
+}
+
+int fdb_tape_stats(const fdb_tape_t* t, int* snaps, int64_t* forward_steps,
+                   int64_t* adjoint_steps) {
+  if (!t) return tfail(FDB_ERR_INVALID, "tape_stats: null tape");
+  if (snaps) *snaps = t->snaps;
+  if (forward_steps) *forward_steps = t->fwd_steps;
+  if (adjoint_steps) *adjoint_steps = t->adj_steps;
+  return FDB_OK;
+}
+
+int fdb_tape_destroy(fdb_tape_t* t) {
+  tape_free(t);
+  return FDB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,86 @@
+// upwindCxx_main.cpp -- drop-in replacement for the reference executable
+// upwindCxx (pletzer/fidibench upwind/cxx/upwind.cxx:139-217) running on
+// the GPU through libfdb_upwind.so.
+//
+// Same options (-numCells, -numSteps, -vtk, -std, -h), same defaults
+// (128 cells, 10 steps), same velocity/lengths (all 1), same dt rule
+// (courant 0.1: dt = min_j 0.1*dx_j/v_j, upwind.cxx:185-192) and the same
+// stdout lines, so the reference's CTest entries (PASS regex
+// "[Cc]heck sum:[ ]*[1|0\\.999]", upwind/cxx/CMakeLists.txt:54-91) run
+// unchanged.  The reference's OpenMP banner lines (upwind.cxx:159-171) are
+// not printed: no OpenMP team runs here, as in a reference build without
+// OpenMP.  Extensions: -precision {f64,f32,bf16}, -timeBlock k.
+#include <iostream>
+#include <limits>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "fidibench_amd/CmdLineArgs.hpp"
+#include "fidibench_amd/Upwind.hpp"
+
+int main(int argc, char** argv) {
+  const int ndims = 3;
+
+  fidibench::CmdLineArgs args;
+  args.setPurpose("Purpose: benchmark finite difference operations.");
+  args.set("-numCells", 128, "Number of cells along each axis");
+  args.set("-numSteps", 10, "Number of time steps");
+  args.set("-vtk", false, "Write output to VTK file");
+  args.set("-std", false, "Print out spread of solution");
+  args.set("-precision", "f64", "Field storage: f64 (bit-exact), f32 or bf16");
+  args.set("-timeBlock", 1, "Time steps fused per pass over the field");
+  args.setFooter("\nfidibench-amd: upwind advection on the GPU\n");
+
+  const bool success = args.parse(argc, argv);
+  const bool help = args.get<bool>("-h");
+  if (!success || help) {
+    if (!success) std::cerr << "ERROR when parsing command line arguments\n";
+    args.help();
+    return 0;
+  }
+
+  const int numTimeSteps = args.get<int>("-numSteps");
+  const bool doVtk = args.get<bool>("-vtk");
+  const bool doStd = args.get<bool>("-std");
+  const std::string prec = args.get<std::string>("-precision");
+  int dtype = FDB_F64;
+  if (prec == "f32") dtype = FDB_F32;
+  else if (prec == "bf16") dtype = FDB_BF16;
+  else if (prec != "f64") {
+    std::cerr << "ERROR: -precision must be f64, f32 or bf16\n";
+    return 1;
+  }
+
+  std::vector<size_t> numCells(ndims, (size_t)args.get<int>("-numCells"));
+  std::cout << "number of cells: ";
+  for (size_t i = 0; i < numCells.size(); ++i) std::cout << ' ' << numCells[i];
+  std::cout << '\n';
+  std::cout << "number of time steps: " << numTimeSteps << '\n';
+
+  std::vector<double> velocity(numCells.size(), 1.0);
+  std::vector<double> lengths(numCells.size(), 1.0);
+
+  const double courant = 0.1;
+  double dt = std::numeric_limits<double>::max();
+  for (size_t j = 0; j < velocity.size(); ++j) {
+    const double dx = lengths[j] / numCells[j];
+    const double val = courant * dx / velocity[j];
+    dt = (val < dt ? val : dt);
+  }
+
+  try {
+    fidibench::Upwind<ndims> up(velocity, lengths, numCells, dtype);
+    up.setTimeBlock(args.get<int>("-timeBlock"));
+    if (doVtk) up.saveVTK("up0.vtk");
+    up.advect(numTimeSteps, dt);
+    std::cout << "check sum: " << up.checksum() << '\n';
+    if (doStd) std::cout << "std      : " << up.std() << '\n';
+    if (doVtk) up.saveVTK("up1.vtk");
+  } catch (const std::exception& e) {
+    std::cout.flush();
+    std::cerr << "ERROR: " << e.what() << '\n';
+    return 1;
+  }
+  return 0;
+}

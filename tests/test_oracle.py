@@ -1,0 +1,248 @@
+"""Pins the CPU oracle (oracle/upwind_oracle.c) before it is trusted as the
+checker of the HIP path.
+
+* FP64 advect/checksum/std: bit-exact against every golden vector made by
+  the reference's own upwind.cxx (tests/golden), and against the compiled
+  reference (oracle/_ref) on extra random cases where it is built.
+* Closed forms and invariants that a dropped term, a wrong sign, index or
+  transposed axis would break: the multinomial solution of a delta field,
+  exact translation equivariance, mirror symmetry, exact scaling by 2,
+  exact conservation for dyadic data, the max principle.
+* Reduced precision: FP32/BF16 steps equal the FP64 step when every
+  intermediate is exactly representable (dyadic data); the bf16 rounding
+  is checked against torch's round-to-nearest-even conversion.
+* Adjoint/gradient: dot-product test <A x, y> = <x, A^T y>, the TLM against
+  centred differences of the forward, and the misfit gradient against the
+  TLM.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from conftest import bits, load_case
+
+
+# ------------------------------------------------------------ golden vectors
+
+def test_oracle_matches_reference_goldens(oracle, manifest):
+    assert len(manifest["small"]) >= 10
+    for name, m in manifest["small"].items():
+        f0, fT = load_case(name)
+        got = oracle.advect(f0, m["velocity"], m["lengths"], float(m["dt"]), m["steps"])
+        assert np.array_equal(bits(got), bits(fT)), name
+        assert repr(oracle.checksum(got)) == m["checksum"], name
+        assert repr(oracle.std(got)) == m["std"], name
+
+
+@pytest.mark.skipif("not __import__('oracle.oracle', fromlist=['x']).have_ref()",
+                    reason="reference harness not built (no reference checkout)")
+@pytest.mark.parametrize("seed", range(4))
+def test_oracle_matches_compiled_reference_random(oracle, seed):
+    rng = np.random.default_rng(seed)
+    nd = int(rng.integers(1, 4))
+    shape = tuple(int(x) for x in rng.integers(1, 23, size=nd))
+    v = tuple(float(x) for x in rng.uniform(-1, 1, size=nd))
+    L = tuple(float(x) for x in rng.uniform(0.5, 2, size=nd))
+    dt = float(rng.uniform(-0.01, 0.02))
+    f0 = rng.standard_normal(shape)
+    steps = int(rng.integers(0, 9))
+    ref, cs, sd = oracle.ref_run(f0, v, L, dt, steps)
+    got = oracle.advect(f0, v, L, dt, steps)
+    assert np.array_equal(bits(got), bits(ref))
+    assert oracle.checksum(got) == cs
+
+
+def test_checksum_is_left_fold(oracle):
+    x = np.array([1e16, 1.0, -1e16, 1.0, 3.0])
+    assert oracle.checksum(x) == (((((0.0 + 1e16) + 1.0) - 1e16) + 1.0) + 3.0)
+    assert oracle.checksum(x) != math.fsum(x)
+
+
+# ------------------------------------------------------------- closed forms
+
+def test_delta_field_is_multinomial(oracle):
+    """v >= 0, T < N: f_T[k] = T!/(k0!k1!k2!(T-|k|)!) nu^k (1-sum nu)^(T-|k|),
+    placed at k (upwind is the lower neighbour, so mass moves to +k)."""
+    n = (12, 12, 12)
+    v = (0.5, 0.3, 0.2)
+    dt = 0.4 / 12
+    nu = [dt * vj / (1.0 / 12) for vj in v]
+    T = 6
+    f0 = np.zeros(n)
+    f0[0, 0, 0] = 1.0
+    got = oracle.advect(f0, v, (1, 1, 1), dt, T)
+    exp = np.zeros(n)
+    rest = 1 - sum(nu)
+    for k0 in range(T + 1):
+        for k1 in range(T + 1 - k0):
+            for k2 in range(T + 1 - k0 - k1):
+                r = T - k0 - k1 - k2
+                c = math.factorial(T) / (math.factorial(k0) * math.factorial(k1) *
+                                         math.factorial(k2) * math.factorial(r))
+                exp[k0, k1, k2] = c * nu[0] ** k0 * nu[1] ** k1 * nu[2] ** k2 * rest ** r
+    assert np.allclose(got, exp, rtol=1e-13, atol=1e-16)
+
+
+def test_negative_velocity_moves_mass_down(oracle):
+    n = (10, 1, 1)
+    f0 = np.zeros(n)
+    f0[5, 0, 0] = 1.0
+    dt = 1.0 / 10  # nu = 1 exactly: pure shift by one cell per step
+    got = oracle.advect(f0, (-1.0, 0.0, 0.0), (1, 1, 1), dt, 3)
+    assert got[2, 0, 0] == 1.0 and got.sum() == 1.0
+
+
+# --------------------------------------------------------------- invariants
+
+def test_translation_equivariance_bit_exact(oracle):
+    n, v = (9, 10, 11), (0.4, -0.7, 0.25)
+    f0 = oracle.uniform(n, 0, 1, seed=3)
+    dt = 0.02
+    a = oracle.advect(f0, v, (1, 1, 1), dt, 6)
+    s = (4, -3, 7)
+    b = oracle.advect(np.roll(f0, s, axis=(0, 1, 2)), v, (1, 1, 1), dt, 6)
+    assert np.array_equal(bits(np.roll(a, s, axis=(0, 1, 2))), bits(b))
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_mirror_symmetry_bit_exact(oracle, axis):
+    """Reversing axis j and negating v_j mirrors the solution exactly
+    (same coefficient, same operation order)."""
+    n, v = (8, 9, 10), [0.3, -0.45, 0.2]
+    f0 = oracle.uniform(n, 0, 1, seed=4)
+    dt = 0.03
+    a = oracle.advect(f0, v, (1, 1, 1), dt, 5)
+    v2 = list(v)
+    v2[axis] = -v2[axis]
+    b = oracle.advect(np.ascontiguousarray(np.flip(f0, axis)), v2, (1, 1, 1), dt, 5)
+    assert np.array_equal(bits(np.flip(a, axis)), bits(b))
+
+
+def test_scaling_by_two_exact(oracle):
+    n, v = (7, 8, 9), (0.3, 0.6, -0.2)
+    f0 = oracle.uniform(n, -1, 1, seed=5)
+    a = oracle.advect(f0, v, (1, 1, 1), 0.02, 4)
+    b = oracle.advect(2.0 * f0, v, (1, 1, 1), 0.02, 4)
+    assert np.array_equal(bits(2.0 * a), bits(b))
+
+
+def test_mass_and_max_principle(oracle):
+    n, v = (16, 16, 16), (0.9, -0.35, 0.5)
+    from fidibench_amd import configs as K
+    dt = K.cfl_dt(n, v)
+    f0 = oracle.uniform(n, 0.0, 1.0, seed=6)
+    got = oracle.advect(f0, v, (1, 1, 1), dt, 20)
+    assert abs(math.fsum(got.ravel()) - math.fsum(f0.ravel())) < 1e-10
+    assert got.min() >= f0.min() - 1e-15 and got.max() <= f0.max() + 1e-15
+
+
+# -------------------------------------------------------- reduced precision
+
+def _dyadic_case():
+    """Coefficients 1/4, 1/8, 1/16 and 0/1 data: every product and sum of one
+    step has at most 6 significant bits, so it is exact in bf16 too."""
+    n = (6, 7, 8)
+    L = (1.0, 1.0, 1.0)
+    # c_j = dt*|v_j|/dx_j; dt = 1/64, dx = 1/n_j -> pick v so c = 2^-2, 2^-3, 2^-4
+    dt = 1.0 / 64
+    v = (16.0 / n[0], -8.0 / n[1], 4.0 / n[2])
+    rng = np.random.default_rng(0)
+    f0 = rng.integers(0, 2, size=n).astype(np.float64)
+    return n, v, L, dt, f0
+
+
+def test_f32_oracle_equals_f64_on_exact_data(oracle):
+    n, v, L, dt, f0 = _dyadic_case()
+    a = oracle.advect(f0, v, L, dt, 2)
+    b = oracle.advect(f0.astype(np.float32), v, L, dt, 2)
+    assert np.array_equal(a, b.astype(np.float64))
+
+
+def test_bf16_oracle_equals_f64_on_exact_data(oracle):
+    n, v, L, dt, f0 = _dyadic_case()
+    a = oracle.advect(f0, v, L, dt, 1)
+    h = oracle.f32_to_bf16(f0.astype(np.float32)).reshape(n)
+    b = oracle.advect(h, v, L, dt, 1)
+    assert np.array_equal(a, oracle.bf16_to_f32(b).astype(np.float64))
+
+
+def test_bf16_rounding_matches_torch_rne(oracle):
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(1)
+    x = np.concatenate([rng.standard_normal(5000).astype(np.float32),
+                        np.array([0.0, -0.0, 1e-40, -3e-39, 3.4e38, np.inf, -np.inf,
+                                  1.00390625, 1.01171875, 1.005859375], dtype=np.float32)])
+    ours = oracle.f32_to_bf16(x)
+    theirs = torch.from_numpy(x).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+    assert np.array_equal(ours, theirs)
+    nan = oracle.f32_to_bf16(np.array([np.nan], dtype=np.float32))[0]
+    assert (nan & 0x7F80) == 0x7F80 and (nan & 0x7F) != 0
+
+
+def test_f32_error_vs_f64_is_small(oracle):
+    n, v = (24, 24, 24), (0.7, -0.4, 0.2)
+    from fidibench_amd import configs as K
+    dt = K.cfl_dt(n, v)
+    f0 = oracle.sphere_noise(n, (1, 1, 1), (0.5, 0.5, 0.5), 0.3, 0.0, 0)
+    a = oracle.advect(f0, v, (1, 1, 1), dt, 30)
+    b = oracle.advect(f0.astype(np.float32), v, (1, 1, 1), dt, 30).astype(np.float64)
+    assert np.abs(a - b).max() / np.abs(a).max() < 1e-5
+
+
+# ------------------------------------------------------- adjoint / gradient
+
+def test_adjoint_dot_product(oracle):
+    n, v, L = (7, 9, 11), (0.5, -0.25, 0.35), (1.0, 1.5, 0.75)
+    dt = 0.01
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(n)
+    y = rng.standard_normal(n)
+    Ax = oracle.advect(x, v, L, dt, 3)
+    ATy = oracle.adjoint(y, v, L, dt, 3)
+    lhs, rhs = float(np.sum(Ax * y)), float(np.sum(x * ATy))
+    assert abs(lhs - rhs) <= 1e-13 * (abs(lhs) + 1)
+
+
+def test_adjoint_differs_from_forward(oracle):
+    """The transpose is not the operator itself (catches a missing flip)."""
+    n, v = (6, 6, 6), (0.5, 0.3, -0.2)
+    y = oracle.uniform(n, 0, 1, seed=8)
+    assert not np.allclose(oracle.adjoint(y, v, (1, 1, 1), 0.02, 1),
+                           oracle.advect(y, v, (1, 1, 1), 0.02, 1))
+
+
+def test_tlm_matches_central_differences(oracle):
+    n, v, L = (6, 7, 8), np.array([0.4, -0.3, 0.25]), (1, 1, 1)
+    dt, T = 0.02, 6
+    rng = np.random.default_rng(3)
+    f0 = rng.uniform(0, 1, n)
+    df0 = rng.standard_normal(n)
+    dv = rng.standard_normal(3)
+    _, dfT = oracle.tlm(f0, df0, v, dv, L, dt, T)
+    eps = 1e-6
+    fp = oracle.advect(f0 + eps * df0, v + eps * dv, L, dt, T)
+    fm = oracle.advect(f0 - eps * df0, v - eps * dv, L, dt, T)
+    fd = (fp - fm) / (2 * eps)
+    assert np.abs(fd - dfT).max() <= 1e-7 * np.abs(dfT).max()
+
+
+def test_misfit_gradient_matches_tlm(oracle):
+    n, L = (6, 7, 8), (1, 1, 1)
+    v = np.array([0.4, -0.3, 0.25])
+    dt, T = 0.02, 5
+    rng = np.random.default_rng(4)
+    f0 = rng.uniform(0, 1, n)
+    target = rng.uniform(0, 1, n)
+    loss, gf, gv = oracle.misfit_grad(f0, target, v, L, dt, T)
+    fT = oracle.advect(f0, v, L, dt, T)
+    assert abs(loss - 0.5 * math.fsum(((fT - target) ** 2).ravel())) < 1e-12 * loss
+    for k in range(3):
+        df0 = rng.standard_normal(n)
+        dv = rng.standard_normal(3)
+        _, dfT = oracle.tlm(f0, df0, v, dv, L, dt, T)
+        lhs = float(np.sum((fT - target) * dfT))
+        rhs = float(np.sum(gf * df0) + np.dot(gv, dv))
+        assert abs(lhs - rhs) <= 1e-12 * (abs(lhs) + 1e-3)
