@@ -441,10 +441,264 @@ __global__ __launch_bounds__(256) void init_kernel(TS* __restrict__ f, InitArgs 
   }
 }
 
+// ---------------------------------------------- one-step forward, rows --
+//
+// The streaming forward kernel.  Block = 64 lanes (axis 2, VEC cells per
+// lane, one 16-byte access) x WY waves (axis 1); every thread holds R
+// consecutive rows of axis 1 in registers, so the axis-1 neighbour is a
+// register of the same thread except for one halo row per plane.  The
+// block marches along axis 0 through `chunk` planes in the direction that
+// makes the axis-0 neighbour the plane it loaded one iteration earlier,
+// and loads plane p+1 while it computes plane p.  HBM traffic is one read
+// and one write of each cell plus the halo rows and one extra plane per
+// chunk, which mostly hit L2 (tiles sharing them run on the same XCD).
+template <int BYTES> struct RawT;
+template <> struct RawT<2> { using T = uint16_t; };
+template <> struct RawT<4> { using T = uint32_t; };
+template <> struct RawT<8> { using T = uint32_t __attribute__((ext_vector_type(2))); };
+template <> struct RawT<16> { using T = uint32_t __attribute__((ext_vector_type(4))); };
+
+template <typename TS, int VEC, bool NT>
+__device__ inline void store_pack_nt(TS* p, const typename St<TS>::M (&in)[VEC]) {
+  Pack<TS, VEC> pk;
+#pragma unroll
+  for (int v = 0; v < VEC; ++v) pk.v[v] = St<TS>::st(in[v]);
+  using R = typename RawT<sizeof(Pack<TS, VEC>)>::T;
+  const R raw = __builtin_bit_cast(R, pk);
+  if constexpr (NT)
+    __builtin_nontemporal_store(raw, reinterpret_cast<R*>(p));
+  else
+    *reinterpret_cast<R*>(p) = raw;
+}
+
+template <typename TS, int VEC, int R, int WY, bool NT>
+__device__ inline void rows_tile(const TS* __restrict__ in, TS* __restrict__ out,
+                                 const MarchArgs& a, const Coef<typename St<TS>::M>& cf,
+                                 int64_t b) {
+  using M = typename St<TS>::M;
+  const int lane = threadIdx.x;
+  const int wave = threadIdx.y;
+
+  const int64_t ti2 = b % a.tiles2;
+  b /= a.tiles2;
+  const int64_t ti1 = b % a.tiles1;
+  const int64_t ch = b / a.tiles1;
+
+  const int64_t i2 = (ti2 * 64 + lane) * VEC;
+  const bool col_ok = i2 < a.n2;
+  const int64_t i2c = col_ok ? i2 : 0;
+  const int64_t r0 = (ti1 * WY + wave) * R;
+  const int64_t nvalid = r0 < a.n1 ? (a.n1 - r0 < R ? a.n1 - r0 : R) : 0;  // rows to store
+
+  // Row offsets within a plane (rows past the end re-read the last row).
+  int64_t roff[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int64_t row = (r0 + r < a.n1) ? r0 + r : a.n1 - 1;
+    roff[r] = row * a.n2;
+  }
+  // The one axis-1 neighbour row outside the thread's rows, and the row
+  // that needs it.
+  const bool up1 = a.dir1 < 0;
+  const int rl = nvalid > 0 ? (int)nvalid - 1 : 0;
+  // (r0 < n1 whenever nvalid > 0, so both wraps stay in [0, n1); a wave
+  // past the last row reads row 0 and stores nothing.)
+  const int64_t hrow =
+      nvalid == 0 ? 0 : (up1 ? wrap(r0 - 1, a.n1) : wrap(r0 + rl + 1, a.n1));
+  const int64_t hoff = hrow * a.n2 + i2c;
+  // Axis-2 neighbour from outside this lane's cells.
+  const bool lo2 = a.dir2 < 0;
+  const bool edge = lo2 ? (lane == 0 || i2c == 0) : (lane == 63 || i2c + VEC >= a.n2);
+  const int64_t i2e = lo2 ? wrap(i2c - 1, a.n2) : wrap(i2c + VEC, a.n2);
+
+  const int64_t plane = a.n1 * a.n2;
+  const int64_t p_lo = ch * a.chunk;
+  const int64_t p_hi = (p_lo + a.chunk < a.n0) ? p_lo + a.chunk : a.n0;
+  const int64_t step = a.dir0 < 0 ? 1 : -1;
+  int64_t p = a.dir0 < 0 ? p_lo : p_hi - 1;
+  const int64_t cnt = p_hi - p_lo;
+
+  M prev[R][VEC];
+  {
+    const TS* base = in + wrap(p + a.dir0, a.n0) * plane + i2c;
+#pragma unroll
+    for (int r = 0; r < R; ++r) load_pack<TS, VEC>(base + roff[r], prev[r]);
+  }
+  M cur[R][VEC], hal[VEC], ed[R];
+  auto load_plane = [&](int64_t pp, M (&c)[R][VEC], M (&h)[VEC], M (&e)[R]) {
+    const TS* base = in + pp * plane;
+#pragma unroll
+    for (int r = 0; r < R; ++r) load_pack<TS, VEC>(base + roff[r] + i2c, c[r]);
+    load_pack<TS, VEC>(base + hoff, h);
+#pragma unroll
+    for (int r = 0; r < R; ++r) e[r] = edge ? St<TS>::ld(base[roff[r] + i2e]) : M(0);
+  };
+  load_plane(p, cur, hal, ed);
+
+  for (int64_t k = 0; k < cnt; ++k) {
+    M nxt[R][VEC], nhal[VEC], ned[R];
+    if (k + 1 < cnt) load_plane(p + step, nxt, nhal, ned);
+    // axis-2 neighbours: shuffles are done by all lanes together
+    M n2[R][VEC];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (lo2) {
+        const M sh = __shfl_up(cur[r][VEC - 1], 1);
+        n2[r][0] = edge ? ed[r] : sh;
+#pragma unroll
+        for (int v = 1; v < VEC; ++v) n2[r][v] = cur[r][v - 1];
+      } else {
+        const M sh = __shfl_down(cur[r][0], 1);
+        n2[r][VEC - 1] = edge ? ed[r] : sh;
+#pragma unroll
+        for (int v = 0; v < VEC - 1; ++v) n2[r][v] = cur[r][v + 1];
+      }
+    }
+    TS* obase = out + p * plane + i2c;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      M o[VEC];
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+        const M x = cur[r][v];
+        M n1;
+        if (up1)
+          n1 = (r == 0) ? hal[v] : cur[r > 0 ? r - 1 : 0][v];
+        else
+          n1 = (r == rl) ? hal[v] : cur[r + 1 < R ? r + 1 : R - 1][v];
+        M g = x;
+        if (a.act0) g = g - cf.c[0] * (prev[r][v] - x);
+        if (a.act1) g = g - cf.c[1] * (n1 - x);
+        if (a.act2) g = g - cf.c[2] * (n2[r][v] - x);
+        o[v] = g;
+        prev[r][v] = x;
+      }
+      if (col_ok && r < nvalid) store_pack_nt<TS, VEC, NT>(obase + roff[r], o);
+    }
+    if (k + 1 < cnt) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) cur[r][v] = nxt[r][v];
+        ed[r] = ned[r];
+      }
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) hal[v] = nhal[v];
+    }
+    p += step;
+  }
+}
+
+// Tiles are numbered axis-2 fastest, then axis 1, then chunk of axis 0.
+// Hardware hands blocks to the 8 XCDs round-robin by block id; each XCD
+// gets a contiguous range of tile numbers so tiles that share halo rows
+// and chunk-boundary planes share that XCD's L2.  With gridDim.x < ntiles
+// the grid is persistent: block (xcd, l) takes tiles start+l, start+l+n8..
+template <typename TS, int VEC, int R, int WY, bool NT>
+__global__ __launch_bounds__(64 * WY) void rows_kernel(const TS* __restrict__ in,
+                                                       TS* __restrict__ out, MarchArgs a,
+                                                       Coef<typename St<TS>::M> cf,
+                                                       int64_t ntiles) {
+  const int64_t nb = gridDim.x;
+  if ((nb & 7) == 0) {
+    const int64_t xcd = blockIdx.x & 7, l = blockIdx.x >> 3, n8 = nb >> 3;
+    const int64_t per = (ntiles + 7) >> 3;
+    const int64_t lo = xcd * per;
+    const int64_t hi = lo + per < ntiles ? lo + per : ntiles;
+    for (int64_t t = lo + l; t < hi; t += n8) rows_tile<TS, VEC, R, WY, NT>(in, out, a, cf, t);
+  } else {
+    for (int64_t t = blockIdx.x; t < ntiles; t += nb)
+      rows_tile<TS, VEC, R, WY, NT>(in, out, a, cf, t);
+  }
+}
+
 // ------------------------------------------------------------ dispatch ---
 
 template <typename TS>
 constexpr int max_vec() { return (int)(16 / sizeof(TS)); }
+
+// Launch shape of the streaming forward kernel.  The default comes from
+// sweeps at 512^3 on MI355X; FDB_ROWS_CFG="R,WY,CHUNK,NT[,GRID]" overrides
+// it (R and WY from the instantiated set below; GRID 0 = one block per
+// tile, else a persistent grid of that many blocks).
+struct RowsCfg {
+  // 512^3 FP64 sweep: R=8 rows x 2 waves, 64-plane chunks, streaming
+  // stores and a persistent grid of 1536 blocks (6 per CU) run at the
+  // plain-copy bandwidth of the device; fewer rows per thread or more
+  // waves per block lose 2-8 %, R=16 loses 25 % (one wave per SIMD).
+  int R = 8, WY = 2, chunk = 64, nt = 1, grid = 1536;
+};
+
+RowsCfg rows_cfg() {
+  static RowsCfg cfg = [] {
+    RowsCfg c;
+    if (const char* s = getenv("FDB_ROWS_CFG")) {
+      int r, w, ch, nt, gr = 0;
+      if (sscanf(s, "%d,%d,%d,%d,%d", &r, &w, &ch, &nt, &gr) >= 4) {
+        c.R = r;
+        c.WY = w;
+        c.chunk = ch;
+        c.nt = nt;
+        c.grid = gr;
+      }
+    }
+    return c;
+  }();
+  return cfg;
+}
+
+template <typename TS>
+hipError_t rows_dispatch(const Grid3& g, const TS* in, TS* out, hipStream_t s) {
+  using M = typename St<TS>::M;
+  const RowsCfg cfg = rows_cfg();
+  MarchArgs a;
+  a.n0 = g.n[0];
+  a.n1 = g.n[1];
+  a.n2 = g.n[2];
+  a.dir0 = g.up[0];
+  a.dir1 = g.up[1];
+  a.dir2 = g.up[2];
+  a.act0 = g.act[0];
+  a.act1 = g.act[1];
+  a.act2 = g.act[2];
+  Coef<M> cf;
+  for (int j = 0; j < 3; ++j) cf.c[j] = (M)g.c[j];
+  int vec = max_vec<TS>();
+  const uintptr_t align_bits = (uintptr_t)in | (uintptr_t)out;
+  while (vec > 1 && ((a.n2 % vec) != 0 || (align_bits % (vec * sizeof(TS))) != 0)) vec >>= 1;
+  const int R = cfg.R, WY = cfg.WY;
+  a.tiles2 = (a.n2 + 64 * vec - 1) / (64 * vec);
+  a.tiles1 = (a.n1 + (int64_t)WY * R - 1) / ((int64_t)WY * R);
+  const int64_t tiles = a.tiles2 * a.tiles1;
+  int64_t chunk = cfg.chunk > 0 ? cfg.chunk : 32;
+  while (chunk > 1 && tiles * ((a.n0 + chunk - 1) / chunk) < 2048) chunk >>= 1;
+  a.chunk = chunk;
+  const int64_t ntiles = tiles * ((a.n0 + chunk - 1) / chunk);
+  const int64_t nb = (cfg.grid > 0 && cfg.grid < ntiles) ? cfg.grid : ntiles;
+  if (nb <= 0 || nb > 0x7fffffffLL) return hipErrorInvalidConfiguration;
+  bool launched = false;
+#define FDB_ROWS(V, RR, W, NTF)                                                              \
+  if constexpr (V <= max_vec<TS>()) {                                                        \
+    if (!launched && vec == V && R == RR && WY == W && (cfg.nt != 0) == NTF) {              \
+      hipLaunchKernelGGL((rows_kernel<TS, V, RR, W, NTF>), dim3((unsigned)nb), dim3(64, W), \
+                         0, s, in, out, a, cf, ntiles);                                      \
+      launched = true;                                                                       \
+    }                                                                                        \
+  }
+#define FDB_ROWS_V(RR, W, NTF) \
+  FDB_ROWS(8, RR, W, NTF) FDB_ROWS(4, RR, W, NTF) FDB_ROWS(2, RR, W, NTF) FDB_ROWS(1, RR, W, NTF)
+  FDB_ROWS_V(8, 2, true)
+  FDB_ROWS_V(8, 1, true)
+  FDB_ROWS_V(8, 4, true)
+  FDB_ROWS_V(16, 1, true)
+  FDB_ROWS_V(16, 2, true)
+  FDB_ROWS_V(4, 4, true)
+  FDB_ROWS_V(8, 2, false)
+#undef FDB_ROWS_V
+#undef FDB_ROWS
+  if (!launched) return hipErrorInvalidValue;
+  return hipGetLastError();
+}
 
 template <typename TS, typename TF, bool ADJ>
 hipError_t march_dispatch(const Grid3& g, const TS* in, TS* out, const int dirs[3],
@@ -503,8 +757,10 @@ hipError_t march_dispatch(const Grid3& g, const TS* in, TS* out, const int dirs[
 
 template <typename TS>
 hipError_t step_typed(const Grid3& g, const void* in, void* out, hipStream_t s) {
-  return march_dispatch<TS, TS, false>(g, (const TS*)in, (TS*)out, g.up, (const TS*)nullptr,
-                                       nullptr, nullptr, s, false);
+  if (getenv("FDB_MARCH_V1"))  // the first one-row-per-thread kernel, for comparison
+    return march_dispatch<TS, TS, false>(g, (const TS*)in, (TS*)out, g.up, (const TS*)nullptr,
+                                         nullptr, nullptr, s, false);
+  return rows_dispatch<TS>(g, (const TS*)in, (TS*)out, s);
 }
 
 }  // namespace
