@@ -1,29 +1,36 @@
 #!/usr/bin/env python3
 """Benchmark of the upwind advection hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps 100] [--warmup 10] [--no-cpu] [--only f64]
+    python bench.py [--gpus 1] [--steps 100] [--warmup 10] [--no-cpu] [--only f64,f32,...]
 
 A step is one time step of Upwind::advect over the whole field.  The
-workload is sphere_noise512: 512^3 cells, FP64, seeded mixed-sign velocity,
-CFL 0.5, sphere + 0.01 noise (fidibench_amd/configs.py).  The field is
-initialised in HBM before the timed region; the timed region is K advect
-steps between two device synchronisations.  Kernel time is measured with
-HIP events on the stream the kernels run on (fdb_upwind_advect_timed).
+headline workload is sphere_noise512: 512^3 cells, FP64, seeded mixed-sign
+velocity, CFL 0.5, sphere + 0.01 noise (fidibench_amd/configs.py).  Fields
+are initialised in HBM before the timed region; the timed region is K
+advect steps between two device synchronisations, and kernel time is
+measured with HIP events on the stream the kernels run on
+(fdb_upwind_advect_timed).
 
-Prints one JSON line: value = FP64 cell-updates/s (the bit-exact path),
-plus FP32 / BF16 forward and forward+adjoint variants, the roofline of the
-dominant kernel and the CPU baseline (the reference's own upwind.cxx,
-compiled by oracle/Makefile, timed on this host's cores on a bounded sample
-of the same workload).
+One JSON line: value = FP64 cell-updates/s (bit-exact with the reference,
+checked against its 512^3 x 100 checksum), plus variants:
+  f32, bf16        the same workload with 4- and 2-byte storage
+  mixed_gauss1024  1024^3 x 200 Gaussian bubble in FP64/FP32/BF16: rates,
+                   max-abs / RMS / max-rel error vs FP64, mass drift
+  fwd_adj_*        forward + adjoint (dL/df0, dL/dv of a misfit) at 512^3
+                   over the same steps, time ratio to the forward alone
+and the roofline of the dominant kernel and the CPU baseline (the
+reference's own upwind.cxx, compiled by oracle/Makefile, timed on this
+host's cores on a bounded sample of the same workload).
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
+import re
 import subprocess
 import sys
-import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -31,6 +38,7 @@ sys.path.insert(0, ROOT)
 METRIC = "cell-updates/s & %HBM roofline at 512³: FP64 plain vs FP32/BF16 vs fwd+adjoint"
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 BYTES = {"f64": 8, "f32": 4, "bf16": 2}
+ALL = ("f64", "f32", "bf16", "mixed", "fwdadj")
 
 
 def parse():
@@ -40,10 +48,17 @@ def parse():
     p.add_argument("--warmup", type=int, default=10)
     p.add_argument("--n", type=int, default=512, help="cells per axis")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    p.add_argument("--only", default="", help="comma list of variants to run")
-    p.add_argument("--time-blocks", default="", help="comma list of k to sweep (FP64)")
+    p.add_argument("--only", default="", help="comma list of: " + ",".join(ALL))
+    p.add_argument("--time-blocks", default="1", help="comma list of k to sweep (FP64)")
     return p.parse_args()
 
+
+def sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+# ---------------------------------------------------------------- CPU leg
 
 def cpu_baseline(w, budget_s=12.0):
     """Time the reference's advect (oracle/_ref) -- or the oracle port when
@@ -78,19 +93,20 @@ print(json.dumps(dict(kind=kind, steps=steps, seconds=t, threads=O.lib().orc_num
     except Exception as e:  # the baseline is reported, never required
         return {"value": None, "unit": "cell-updates/s", "cores": ncores, "kind": None,
                 "sample": f"failed: {e}"}
-    cells = w.ntot * r["steps"]
     model = ""
     try:
         with open("/proc/cpuinfo") as fh:
             model = next(l.split(":", 1)[1].strip() for l in fh if l.startswith("model name"))
     except Exception:
         pass
-    return {"value": cells / r["seconds"], "unit": "cell-updates/s", "cores": r["threads"],
-            "kind": r["kind"],
+    return {"value": w.ntot * r["steps"] / r["seconds"], "unit": "cell-updates/s",
+            "cores": r["threads"], "kind": r["kind"],
             "sample": (f"{w.num_cells[0]}^3 x {r['steps']} steps of the same workload "
                        f"({r['seconds']:.2f} s, advect only, OMP_NUM_THREADS={r['threads']}, "
                        f"{model})")}
 
+
+# ---------------------------------------------------------------- GPU legs
 
 def run_forward(w, dtype, steps, warmup, time_block):
     """K timed steps on a fresh field; returns (ms_total, launches, handle)."""
@@ -107,9 +123,89 @@ def run_forward(w, dtype, steps, warmup, time_block):
     return ms, launches, up
 
 
-def sync():
+def compare(a_up, b_up):
+    """{max_abs, rms, max_rel} of handle a's field against handle b's."""
+    from fidibench_amd import _lib as L
+    out = (C.c_double * 4)()
+    L.check(L.lib().fdb_compare_dev(a_up.dtype, a_up.device_ptr(), b_up.dtype, b_up.device_ptr(),
+                                    a_up.ntot, out, None))
+    return {"max_abs": out[0], "rms": out[1], "max_rel": out[3]}
+
+
+def mixed_precision(n=1024, steps=200):
+    """The Gaussian-bubble workload in FP64, FP32 and BF16 storage."""
+    from fidibench_amd import configs as K
+    w = K.gauss1024(n=n, steps=steps)
+    res, ups = {}, {}
+    for dt_name in ("f64", "f32", "bf16"):
+        from fidibench_amd import Upwind
+        up = Upwind(w.velocity, w.lengths, w.num_cells, dtype=dt_name)
+        up.init_field(w.init_kind, w.init_params, seed=w.init_seed)
+        m0 = up.mass()
+        sync()
+        ms, launches = up.advect_timed(steps, w.dt)
+        m1 = up.mass()
+        res[dt_name] = {"cell_updates_per_s": w.ntot * steps / (ms * 1e-3), "ms": ms,
+                        "hbm_frac_compulsory": w.ntot * steps / (ms * 1e-3) * 2 *
+                        BYTES[dt_name] / HBM_PEAK,
+                        "mass_drift_rel": abs(m1 - m0) / abs(m0)}
+        ups[dt_name] = up
+    for dt_name in ("f32", "bf16"):
+        res[dt_name]["error_vs_f64"] = compare(ups[dt_name], ups["f64"])
+    for up in ups.values():
+        up.close()
+    res["workload"] = (f"gauss{n}: {n}^3, {steps} steps, v=(0.7,-0.4,0.2), CFL 0.5, "
+                       "exp(-|x-c|^2/0.005), seeded centre")
+    return res
+
+
+def forward_adjoint(w, steps, f_dtype, fwd_ms):
+    """Forward (keeping checkpoints) + backward of a misfit loss over
+    `steps` steps at the workload's size; time ratio to the forward alone."""
     import torch
-    torch.cuda.synchronize()
+    from fidibench_amd import _lib as L
+    from fidibench_amd.autograd import Tape, default_budget
+    from fidibench_amd import configs as K
+    tdt = {"f64": torch.float64, "f32": torch.float32}[f_dtype]
+    fd = {"f64": L.F64, "f32": L.F32}[f_dtype]
+    f0 = torch.empty(w.num_cells, dtype=tdt, device="cuda")
+    L.check(L.lib().fdb_init_field_dev(3, L.u64s(w.num_cells), L.dbls(w.lengths), fd, w.init_kind,
+                                       L.dbls(w.init_params), len(w.init_params), w.init_seed,
+                                       f0.data_ptr(), torch.cuda.current_stream().cuda_stream))
+    fT = torch.empty_like(f0)
+    lam0 = torch.empty_like(f0)
+    tape = Tape(w.num_cells, w.velocity, w.lengths, w.dt, steps, fd, fd, default_budget(0.8))
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    sync()
+    e0.record()
+    tape.forward(f0, fT)
+    e1.record()
+    lamT = fT.clone()  # dL/df_T of 0.5*|f_T|^2, a stand-in misfit gradient
+    gv = tape.backward(lamT, lam0)
+    e2.record()
+    sync()
+    fwd = e0.elapsed_time(e1)
+    tot = e0.elapsed_time(e2)
+    st = tape.stats()
+    tape.close()
+    return {"ms_forward_with_checkpoints": fwd, "ms_forward_plus_adjoint": tot,
+            "ratio_to_plain_forward": tot / fwd_ms if fwd_ms else None,
+            "cell_steps_per_s": w.ntot * steps / (tot * 1e-3), "snapshots": st["snapshots"],
+            "forward_steps": st["forward_steps"], "adjoint_steps": st["adjoint_steps"],
+            "grad_v": gv}
+
+
+def pmc_traffic(kernel_key):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, if any."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.txt")
+    try:
+        for line in open(path):
+            m = re.match(r"\s*" + re.escape(kernel_key) + r"\s*:\s*([0-9.eE+]+)", line)
+            if m:
+                return float(m.group(1))
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -118,19 +214,16 @@ def main():
         print(f"bench.py measures one GPU only (got --gpus {a.gpus}): the single-GPU "
               "build has no sharded path to run on more.", file=sys.stderr)
         sys.exit(2)
-    import torch  # noqa: F401  (device sync around the timed region)
+    import torch  # noqa: F401  (device sync around the timed region; loads the HIP runtime)
     from fidibench_amd import configs as K
 
     w = K.sphere_noise512(n=a.n, steps=a.steps)
-    only = set(a.only.split(",")) if a.only else {"f64", "f32", "bf16"}
+    only = set(a.only.split(",")) if a.only else set(ALL)
     res = {}
-    kernel = {}
+    best = None
 
-    # FP64: the bit-exact path; sweep the temporal-blocking depth.
-    ks = [int(x) for x in a.time_blocks.split(",")] if a.time_blocks else [1]
     if "f64" in only:
-        best = None
-        for k in ks:
+        for k in [int(x) for x in a.time_blocks.split(",")]:
             ms, launches, up = run_forward(w, "f64", a.steps, a.warmup, k)
             cu = w.ntot * a.steps / (ms * 1e-3)
             entry = {"cell_updates_per_s": cu, "ms": ms, "launches": launches,
@@ -141,24 +234,31 @@ def main():
             if best is None or cu > best[0]:
                 best = (cu, k, ms, launches)
             up.close()
-        kernel["f64"] = best
     for dt_name in ("f32", "bf16"):
         if dt_name in only:
             ms, launches, up = run_forward(w, dt_name, a.steps, a.warmup, 1)
             cu = w.ntot * a.steps / (ms * 1e-3)
-            m0 = None
             res[dt_name] = {"cell_updates_per_s": cu, "ms": ms, "launches": launches,
                             "hbm_frac_compulsory": cu * 2 * BYTES[dt_name] / HBM_PEAK}
+            if best:
+                res[dt_name]["speedup_vs_f64"] = cu / best[0]
             up.close()
+    if "mixed" in only:
+        res["mixed_gauss1024"] = mixed_precision()
+    if "fwdadj" in only:
+        fwd_ms = best[2] if best else None
+        res["fwd_adj_f64"] = forward_adjoint(w, a.steps, "f64", fwd_ms)
+        f32_ms = res.get("f32", {}).get("ms")
+        res["fwd_adj_f32"] = forward_adjoint(w, a.steps, "f32", f32_ms)
 
-    # Parity of the timed run with the reference (golden checksum).
     parity = None
     try:
         with open(os.path.join(ROOT, "tests", "golden", "manifest.json")) as fh:
             gold = json.load(fh)["large"].get("sphere_noise512")
-        got = [v.get("checksum") for k, v in res.items() if k.startswith("f64_")]
+        got = [v.get("checksum") for k, v in res.items() if k.startswith("f64_k")]
         if gold and got and all(g is not None for g in got):
             parity = {"reference_checksum": gold["checksum"],
+                      "checksums": [repr(g) for g in got],
                       "bit_exact": all(repr(g) == gold["checksum"] for g in got)}
     except Exception:
         pass
@@ -166,8 +266,8 @@ def main():
     out = {"metric": METRIC, "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps,
            "warmup": a.warmup, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic"}
-    if "f64" in kernel:
-        cu, k, ms, launches = kernel["f64"]
+    if best:
+        cu, k, ms, launches = best
         out["value"] = cu
         out["ms_per_step"] = ms / a.steps
         per_launch_ms = ms / launches
@@ -175,8 +275,9 @@ def main():
         alg_bytes = 16.0 * w.ntot * steps_per_launch
         achieved = alg_bytes / (per_launch_ms * 1e-3)
         out["roofline"] = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                           "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
-                           "kernel": "march_kernel<double> (FP64 forward)",
+                           "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                           "traffic": pmc_traffic(f"rows_kernel_f64_k{k}"),
+                           "kernel": f"rows_kernel<double> (FP64 forward, {k} step/launch)",
                            "algorithmic_bytes_per_launch": alg_bytes,
                            "avg_launch_ms": per_launch_ms}
     out["config"] = {"workload": f"sphere_noise{a.n}: {a.n}^3 cells, {a.steps} steps, FP64, "

@@ -60,6 +60,7 @@ SIGNATURES = {
     "fdb_tape_stats": (_i, [_vp, C.POINTER(_i), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "fdb_tape_destroy": (_i, [_vp]),
     "fdb_init_field_dev": (_i, [_i, _u64p, _dp, _i, _i, _dp, _i, C.c_uint64, _vp, _vp]),
+    "fdb_compare_dev": (_i, [_i, _vp, _i, _vp, C.c_uint64, _dp, _vp]),
     "fdb_host_checksum": (C.c_double, [_dp, C.c_uint64]),
     "fdb_host_std": (C.c_double, [_dp, C.c_uint64]),
     "fdb_host_save_vtk": (_i, [_i, _u64p, _dp, _dp, C.c_char_p]),

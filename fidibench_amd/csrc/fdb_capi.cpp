@@ -490,6 +490,34 @@ int fdb_misfit_dev(int f_dtype, const void* f, const void* target, uint64_t n, i
   return FDB_OK;
 }
 
+int fdb_compare_dev(int a_dtype, const void* a, int b_dtype, const void* b, uint64_t n,
+                    double* out4, void* stream) {
+  if (!valid_dtype(a_dtype) || !valid_dtype(b_dtype) || !a || !b || !out4)
+    return fail(FDB_ERR_INVALID, "compare_dev: bad argument");
+  double* part = nullptr;
+  if (int rc = dev_scratch((6 * fdb::kReduceBlocks + 6) * sizeof(double), &part)) return rc;
+  FDB_HIP(fdb::launch_compare(a_dtype, a, b_dtype, b, n, part, (hipStream_t)stream));
+  std::vector<double> h(6 * (size_t)fdb::kReduceBlocks);
+  FDB_HIP(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                         (hipStream_t)stream));
+  FDB_HIP(hipStreamSynchronize((hipStream_t)stream));
+  double mx = 0.0, s = 0.0, c = 0.0, mb = 0.0;
+  for (int i = 0; i < fdb::kReduceBlocks; ++i) {
+    const double* p = &h[6 * (size_t)i];
+    mx = std::max(mx, p[0]);
+    mb = std::max(mb, p[3]);
+    const double sum = s + p[1];
+    const double bb = sum - s;
+    c += (s - (sum - bb)) + (p[1] - bb) + p[2];
+    s = sum;
+  }
+  out4[0] = mx;
+  out4[1] = n ? std::sqrt((s + c) / (double)n) : 0.0;
+  out4[2] = mb;
+  out4[3] = mb > 0.0 ? mx / mb : 0.0;
+  return FDB_OK;
+}
+
 int fdb_init_field_dev(int ndims, const uint64_t* num_cells, const double* lengths, int dtype,
                        int kind, const double* params, int nparams, uint64_t seed, void* f,
                        void* stream) {

@@ -78,6 +78,11 @@ hipError_t launch_init(int dtype, const Grid3& g, const double* len, int kind,
 hipError_t launch_convert(int src_dtype, const void* src, int dst_dtype, void* dst, uint64_t n,
                           hipStream_t s);
 
+// Per-block {max|a-b|, sum, comp of sum (a-b)^2, max|b|} at
+// partials[block*6 + 0..3] for kReduceBlocks blocks.
+hipError_t launch_compare(int a_dtype, const void* a, int b_dtype, const void* b, uint64_t n,
+                          double* partials, hipStream_t s);
+
 // Zero the 6-double accumulator with a kernel (no memset nodes, so the
 // call stays graph-capturable).
 hipError_t launch_zero(double* p, int n, hipStream_t s);

@@ -355,6 +355,45 @@ __global__ __launch_bounds__(256) void misfit_kernel(const TF* __restrict__ f,
   if (t < 2) partials[blockIdx.x * 6 + t] = red[0][t];
 }
 
+// Per-block {max|a-b|, sum (a-b)^2 (Neumaier pair), max|b|} of two fields
+// widened to double -> partials[block*6 + 0..3].
+template <typename TA, typename TB>
+__global__ __launch_bounds__(256) void compare_kernel(const TA* __restrict__ a,
+                                                      const TB* __restrict__ b, uint64_t n,
+                                                      double* __restrict__ partials) {
+  __shared__ double red[256][4];
+  const int t = threadIdx.x;
+  double mx = 0.0, s = 0.0, e = 0.0, mb = 0.0;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + t; i < n; i += (uint64_t)gridDim.x * 256) {
+    const double x = widen(a, i), y = widen(b, i);
+    const double d = x - y;
+    mx = fmax(mx, __builtin_fabs(d));
+    mb = fmax(mb, __builtin_fabs(y));
+    const double q = d * d;
+    const double sum = s + q;
+    e += (s >= q) ? (s - sum) + q : (q - sum) + s;
+    s = sum;
+  }
+  red[t][0] = mx;
+  red[t][1] = s;
+  red[t][2] = e;
+  red[t][3] = mb;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      red[t][0] = fmax(red[t][0], red[t + w][0]);
+      const double s1 = red[t][1], s2 = red[t + w][1];
+      const double sum = s1 + s2;
+      const double bb = sum - s1;
+      red[t][2] = red[t][2] + red[t + w][2] + ((s1 - (sum - bb)) + (s2 - bb));
+      red[t][1] = sum;
+      red[t][3] = fmax(red[t][3], red[t + w][3]);
+    }
+    __syncthreads();
+  }
+  if (t < 4) partials[blockIdx.x * 6 + t] = red[0][t];
+}
+
 __global__ void zero_kernel(double* p, int n) {
   const int t = threadIdx.x;
   if (t < n) p[t] = 0.0;
@@ -947,6 +986,25 @@ hipError_t launch_convert(int sd, const void* src, int dd, void* dst, uint64_t n
   else if (sd == FDB_BF16 && dd == FDB_F32) FDB_CV(uint16_t, float);
   else return hipErrorInvalidValue;
 #undef FDB_CV
+  return hipGetLastError();
+}
+
+hipError_t launch_compare(int da, const void* a, int db, const void* b, uint64_t n,
+                          double* partials, hipStream_t s) {
+  const dim3 grid(kReduceBlocks), block(256);
+#define FDB_CMP(TA, TB) \
+  hipLaunchKernelGGL((compare_kernel<TA, TB>), grid, block, 0, s, (const TA*)a, (const TB*)b, n, partials)
+#define FDB_CMP_B(TA)                                 \
+  if (db == FDB_F64) FDB_CMP(TA, double);            \
+  else if (db == FDB_F32) FDB_CMP(TA, float);        \
+  else if (db == FDB_BF16) FDB_CMP(TA, uint16_t);    \
+  else return hipErrorInvalidValue;
+  if (da == FDB_F64) { FDB_CMP_B(double) }
+  else if (da == FDB_F32) { FDB_CMP_B(float) }
+  else if (da == FDB_BF16) { FDB_CMP_B(uint16_t) }
+  else return hipErrorInvalidValue;
+#undef FDB_CMP_B
+#undef FDB_CMP
   return hipGetLastError();
 }
 

@@ -166,6 +166,12 @@ int fdb_tape_stats(const fdb_tape_t* t, int* snaps, int64_t* forward_steps,
                    int64_t* adjoint_steps);
 int fdb_tape_destroy(fdb_tape_t* t);
 
+/* Error of field a against reference field b (n values each, any dtypes,
+ * widened to double): out4 = {max|a-b|, sqrt(mean((a-b)^2)), max|b|,
+ * max|a-b| / max|b|}.  Blocks. */
+int fdb_compare_dev(int a_dtype, const void* a, int b_dtype, const void* b, uint64_t n,
+                    double* out4, void* stream);
+
 /* Fill a device field (dtype) with a synthetic initial condition. */
 int fdb_init_field_dev(int ndims, const uint64_t* num_cells, const double* lengths, int dtype,
                        int kind, const double* params, int nparams, uint64_t seed, void* f,

@@ -1,0 +1,210 @@
+"""The C ABI on the host: exported symbols, host-side helpers (checksum, std,
+saveVTK bytes), the checkpoint schedule, argument errors and the absence of
+fused multiply-adds in the FP64 kernels.  Runs without a GPU."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+import subprocess
+from math import comb
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT, load_case
+
+HEADER = os.path.join(ROOT, "include", "fidibench_amd", "fdb_upwind.h")
+
+
+def _lib():
+    from fidibench_amd import _lib as L
+    return L
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(fdb_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_every_declared_symbol_is_exported():
+    L = _lib()
+    lib = L.lib()
+    names = declared_functions()
+    assert len(names) >= 30
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(L.SIGNATURES), set(names) ^ set(L.SIGNATURES)
+
+
+def test_library_has_gfx950_code_and_no_torch_dependency():
+    L = _lib()
+    out = subprocess.run(["ldd", L.LIB_PATH], capture_output=True, text=True).stdout
+    assert "libtorch" not in out and "libc10" not in out
+    blob = open(L.LIB_PATH, "rb").read()
+    assert b"gfx950" in blob
+
+
+def test_host_checksum_and_std_match_reference(manifest):
+    L = _lib()
+    for name, m in manifest["small"].items():
+        _, fT = load_case(name)
+        f = np.ascontiguousarray(fT, dtype=np.float64)
+        p = f.ctypes.data_as(C.POINTER(C.c_double))
+        assert repr(L.lib().fdb_host_checksum(p, f.size)) == m["checksum"], name
+        assert repr(L.lib().fdb_host_std(p, f.size)) == m["std"], name
+
+
+@pytest.mark.parametrize("name", ["vtk3d", "vtk2d", "vtk1d"])
+def test_save_vtk_bytes_match_reference(manifest, tmp_path, name):
+    L = _lib()
+    m = manifest["vtk"][name]
+    with np.load(os.path.join(GOLDEN, f"{name}.npz"), allow_pickle=False) as z:
+        fT = np.ascontiguousarray(z["fT"])
+    out = tmp_path / "o.vtk"
+    L.check(L.lib().fdb_host_save_vtk(len(m["shape"]), L.u64s(m["shape"]), L.dbls(m["lengths"]),
+                                      fT.ctypes.data_as(C.POINTER(C.c_double)),
+                                      str(out).encode()))
+    assert out.read_bytes() == open(os.path.join(GOLDEN, m["file"]), "rb").read()
+
+
+def test_save_vtk_unwritable_path_is_io_error():
+    L = _lib()
+    f = np.zeros(8)
+    rc = L.lib().fdb_host_save_vtk(1, L.u64s([8]), L.dbls([1.0]),
+                                   f.ctypes.data_as(C.POINTER(C.c_double)),
+                                   b"/nonexistent_dir/x.vtk")
+    assert rc == L.ERR_IO and b"cannot open" in L.lib().fdb_last_error()
+
+
+# ------------------------------------------------------ checkpoint schedule
+
+def _plan(T, snaps):
+    L = _lib()
+    cap = 20 * T * T + 100
+    acts = (C.c_int * cap)()
+    args = (C.c_int * cap)()
+    fw = C.c_int64()
+    n = L.lib().fdb_ckpt_plan(T, snaps, acts, args, cap, C.byref(fw))
+    assert 0 <= n <= cap
+    return [(acts[i], args[i]) for i in range(n)], fw.value
+
+
+def _simulate(T, snaps, plan):
+    """Run the schedule on integer 'states' and check every adjoint step
+    sees the right state, once, in reverse order, within the snapshot
+    budget.  Returns the number of forward steps."""
+    L = _lib()
+    stack = [0]
+    work = 0
+    fwd = 0
+    seen = []
+    for a, arg in plan:
+        if a == L.CK_ADVANCE:
+            assert arg >= 1
+            work += arg
+            fwd += arg
+        elif a == L.CK_STORE:
+            stack.append(work)
+            assert len(stack) - 1 <= snaps
+        elif a == L.CK_RESTORE:
+            work = stack[-1]
+        elif a == L.CK_POP:
+            stack.pop()
+            assert stack
+        elif a == L.CK_ADJOINT:
+            assert work == arg
+            seen.append(arg)
+        else:
+            raise AssertionError(a)
+    assert seen == list(range(T - 1, -1, -1))
+    return fwd
+
+
+def _brute(T, snaps, memo={}):
+    """Optimal forward-step count of the recursive schedule, by DP."""
+    if (T, snaps) in memo:
+        return memo[(T, snaps)]
+    if T == 1:
+        r = 0
+    elif snaps == 0:
+        r = T * (T - 1) // 2
+    else:
+        r = min(m + _brute(T - m, snaps - 1) + _brute(m, snaps) for m in range(1, T))
+    memo[(T, snaps)] = r
+    return r
+
+
+@pytest.mark.parametrize("T", [1, 2, 3, 7, 16, 33, 50])
+@pytest.mark.parametrize("snaps", [0, 1, 2, 3, 5, 60])
+def test_checkpoint_plan_is_valid_and_optimal(T, snaps):
+    plan, fw = _plan(T, snaps)
+    assert _simulate(T, min(snaps, max(T - 1, 0)), plan) == fw
+    assert fw == _brute(T, min(snaps, T - 1))
+    if 0 < snaps < T - 1:
+        s = snaps + 1
+        r = 0
+        while comb(s + r, s) < T:
+            r += 1
+        assert fw == r * T - comb(s + r, s + 1)
+
+
+def test_large_plan_repetition_two():
+    """400 steps with 165 snapshots (the 1024x1024x768 BF16 case) needs r=2."""
+    plan, fw = _plan(400, 165)
+    assert _simulate(400, 165, plan) == fw
+    assert fw == 2 * 400 - comb(168, 167)
+
+
+# ------------------------------------------------------------ arguments
+
+def test_invalid_arguments_are_reported():
+    L = _lib()
+    h = C.c_void_p()
+    rc = L.lib().fdb_upwind_create(4, L.dbls([1] * 4), L.dbls([1] * 4), L.u64s([2] * 4), 0,
+                                   C.byref(h))
+    assert rc == L.ERR_INVALID and b"ndims" in L.lib().fdb_last_error()
+    rc = L.lib().fdb_upwind_create(3, L.dbls([1] * 3), L.dbls([1] * 3), L.u64s([2, 0, 2]), 0,
+                                   C.byref(h))
+    assert rc == L.ERR_INVALID
+    rc = L.lib().fdb_upwind_create(3, L.dbls([1] * 3), L.dbls([1] * 3), L.u64s([2] * 3), 7,
+                                   C.byref(h))
+    assert rc == L.ERR_INVALID and b"dtype" in L.lib().fdb_last_error()
+    rc = L.lib().fdb_upwind_advect(None, 3, 0.1)
+    assert rc == L.ERR_INVALID
+    rc = L.lib().fdb_tape_create(3, L.u64s([4, 4, 4]), L.dbls([1] * 3), L.dbls([1] * 3), 0.1, 0,
+                                 0, 0, 1 << 20, C.byref(h))
+    assert rc == L.ERR_INVALID
+
+
+def test_create_without_gpu_is_nodev():
+    torch = pytest.importorskip("torch")
+    if torch.cuda.device_count() > 0:
+        pytest.skip("a GPU is present")
+    L = _lib()
+    h = C.c_void_p()
+    rc = L.lib().fdb_upwind_create(3, L.dbls([1] * 3), L.dbls([1] * 3), L.u64s([8] * 3), 0,
+                                   C.byref(h))
+    assert rc == L.ERR_NODEV
+
+
+# --------------------------------------------------------- code generation
+
+def test_fp64_kernels_have_no_fused_multiply_add(tmp_path):
+    """-ffp-contract=off must hold: no v_fma_f64 / v_fmac_f64 in the FP64
+    forward kernels, and no packed FP32 instructions anywhere."""
+    src = os.path.join(ROOT, "fidibench_amd", "csrc", "fdb_kernels.hip")
+    asm = tmp_path / "k.s"
+    subprocess.run(["/opt/rocm/bin/hipcc", "--offload-arch=gfx950", "-O3", "-std=c++17",
+                    "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang",
+                    "-packed-fp32-ops", "--cuda-device-only", "-S", "-o", str(asm), src],
+                   check=True, capture_output=True)
+    text = asm.read_text()
+    bodies = re.findall(r"^(_ZN3fdb\S*(?:rows|march)_kernelId\S*):[^\n]*\n(.*?)s_endpgm", text,
+                        re.M | re.S)
+    fwd = [(n, b) for n, b in bodies if "Lb0E" in n or "rows_kernel" in n]
+    assert len(fwd) >= 4
+    for name, body in fwd:
+        assert not re.search(r"v_fmac?_f64", body), name
+    assert not re.search(r"v_pk_(add|mul|fma)_f32", text)

@@ -1,0 +1,189 @@
+"""Adjoint, velocity gradient, checkpointing and the autograd harness on the GPU.
+
+The adjoint step is the exact transpose of the forward step; its FP64
+kernel follows the oracle's operation order, so lam is compared bit for
+bit.  Velocity gradients are compensated FP64 reductions compared with the
+oracle's long-double sums.  The full gradient is checked against centred
+finite differences of the GPU forward (eps = 1e-6, FP64) on seeded random
+directions, against the oracle's FP64 tangent-linear model, and the
+checkpointed sweep against the store-everything sweep (bit for bit).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import bits
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _setup(shape=(10, 12, 70), v=(0.45, -0.3, 0.2), T=6, seed=0):
+    from fidibench_amd import configs as K
+    from oracle import oracle as O
+    dt = K.cfl_dt(shape, v)
+    f0 = O.uniform(shape, 0.0, 1.0, seed=seed)
+    target = O.uniform(shape, 0.0, 1.0, seed=seed + 1)
+    return shape, v, dt, T, f0, target
+
+
+def test_adjoint_step_bit_exact_vs_oracle(oracle):
+    from fidibench_amd import _lib as L
+    for shape, v in [((10, 12, 70), (0.45, -0.3, 0.2)), ((7, 9, 130), (-0.2, 0.6, -0.5)),
+                     ((5, 1, 33), (0.3, 0.0, -0.7))]:
+        lam_next = oracle.uniform(shape, -1, 1, seed=3)
+        ref = oracle.adjoint(lam_next, v, (1, 1, 1), 0.013, 1)
+        a = torch.from_numpy(lam_next).cuda()
+        b = torch.empty_like(a)
+        L.check(L.lib().fdb_adjoint_step_dev(3, L.u64s(shape), L.dbls(v), L.dbls((1, 1, 1)), 0.013,
+                                             L.F64, a.data_ptr(), b.data_ptr(), L.F64, None, None,
+                                             torch.cuda.current_stream().cuda_stream))
+        torch.cuda.synchronize()
+        assert np.array_equal(bits(b.cpu().numpy()), bits(ref)), shape
+
+
+def test_dot_product_identity_on_gpu():
+    from fidibench_amd import _lib as L
+    shape, v, dt = (16, 20, 130), (0.3, -0.45, 0.25), 0.002
+    g = torch.Generator().manual_seed(0)
+    x = torch.randn(shape, dtype=torch.float64, generator=g).cuda()
+    y = torch.randn(shape, dtype=torch.float64, generator=g).cuda()
+    a, b = x.clone(), torch.empty_like(x)
+    res = C.c_int()
+    s = torch.cuda.current_stream().cuda_stream
+    L.check(L.lib().fdb_advect_dev(3, L.u64s(shape), L.dbls(v), L.dbls((1, 1, 1)), dt, L.F64,
+                                   a.data_ptr(), b.data_ptr(), 1, 1, s, C.byref(res)))
+    Ax = b if res.value else a
+    ATy = torch.empty_like(y)
+    L.check(L.lib().fdb_adjoint_step_dev(3, L.u64s(shape), L.dbls(v), L.dbls((1, 1, 1)), dt, L.F64,
+                                         y.data_ptr(), ATy.data_ptr(), L.F64, None, None, s))
+    lhs = float((Ax * y).sum())
+    rhs = float((x * ATy).sum())
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+def _tape_grad(f0, target, v, dt, T, max_bytes=None, f_dtype=torch.float64):
+    from fidibench_amd.autograd import advect, misfit
+    f0t = torch.from_numpy(f0).cuda().to(f_dtype).requires_grad_(True)
+    vt = torch.tensor(v, dtype=torch.float64, requires_grad=True)
+    fT = advect(f0t, vt, dt, T, max_bytes=max_bytes)
+    loss = misfit(fT, torch.from_numpy(target).cuda().to(f_dtype))
+    loss.backward()
+    return float(loss), f0t.grad.double().cpu().numpy(), vt.grad.numpy().copy(), fT
+
+
+def test_gradient_matches_oracle(oracle):
+    shape, v, dt, T, f0, target = _setup()
+    loss, gf, gv, fT = _tape_grad(f0, target, v, dt, T)
+    rl, rgf, rgv = oracle.misfit_grad(f0, target, v, (1, 1, 1), dt, T)
+    assert np.array_equal(bits(fT.detach().cpu().numpy()),
+                          bits(oracle.advect(f0, v, (1, 1, 1), dt, T)))
+    assert abs(loss - rl) <= 1e-14 * rl
+    assert np.array_equal(bits(gf), bits(rgf))  # same adjoint op order: bit-exact
+    assert np.allclose(gv, rgv, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("snaps", [0, 1, 3])
+def test_checkpointing_is_exact(snaps):
+    """Few snapshots (recomputation) gives the same bits as storing all."""
+    shape, v, dt, T, f0, target = _setup(shape=(12, 16, 64), T=23, seed=5)
+    full = _tape_grad(f0, target, v, dt, T)
+    fbytes = int(np.prod(shape)) * 8
+    # fixed working set: initial state, two working buffers, one adjoint buffer
+    part = _tape_grad(f0, target, v, dt, T, max_bytes=(4 + snaps) * fbytes)
+    assert np.array_equal(bits(full[1]), bits(part[1]))
+    assert np.array_equal(bits(full[2]), bits(part[2]))
+
+
+def test_tape_forward_steps_follow_binomial_optimum():
+    from math import comb
+    from fidibench_amd import _lib as L
+    from fidibench_amd.autograd import Tape
+    shape, v, dt, T = (8, 8, 64), (0.3, 0.2, -0.1), 0.001, 40
+    fbytes = int(np.prod(shape)) * 8
+    for snaps in (1, 2, 3, 5, 39):
+        tape = Tape(shape, v, (1, 1, 1), dt, T, L.F64, L.F64, (3 + snaps) * fbytes + fbytes)
+        f0 = torch.rand(shape, dtype=torch.float64, device="cuda")
+        fT = torch.empty_like(f0)
+        tape.forward(f0, fT)
+        lam = torch.rand_like(f0)
+        lam0 = torch.empty_like(f0)
+        tape.backward(lam, lam0)
+        st = tape.stats()
+        assert st["snapshots"] == snaps and st["adjoint_steps"] == T
+        s = snaps + 1
+        r = 0
+        while comb(s + r, s) < T:
+            r += 1
+        assert st["forward_steps"] == r * T - comb(s + r, s + 1) + 1, (snaps, st)
+        tape.close()
+
+
+def test_gradient_vs_central_differences(oracle):
+    """FP64: <grad, d> against (L(x+eps d) - L(x-eps d)) / (2 eps), eps = 1e-6,
+    on 20 seeded random directions in (f0, v)."""
+    from fidibench_amd.autograd import advect, misfit
+    shape, v, dt, T, f0, target = _setup(shape=(16, 16, 64), T=12, seed=9)
+    loss, gf, gv, _ = _tape_grad(f0, target, v, dt, T)
+    f0t = torch.from_numpy(f0).cuda()
+    tg = torch.from_numpy(target).cuda()
+    vt = torch.tensor(v, dtype=torch.float64)
+    rng = np.random.default_rng(20)
+    eps = 1e-6
+
+    def L_(f, vv):
+        with torch.no_grad():
+            return float(misfit(advect(f, vv, dt, T), tg))
+
+    worst = 0.0
+    for _ in range(20):
+        df = rng.standard_normal(shape)
+        dv = rng.standard_normal(3)
+        dft = torch.from_numpy(df).cuda()
+        dvt = torch.from_numpy(dv)
+        fd = (L_(f0t + eps * dft, vt + eps * dvt) - L_(f0t - eps * dft, vt - eps * dvt)) / (2 * eps)
+        an = float(np.sum(gf * df) + np.dot(gv, dv))
+        worst = max(worst, abs(fd - an) / abs(an))
+    assert worst < 1e-6, worst
+
+
+def test_gradient_vs_tlm(oracle):
+    shape, v, dt, T, f0, target = _setup(shape=(9, 11, 40), T=8, seed=13)
+    loss, gf, gv, fT = _tape_grad(f0, target, v, dt, T)
+    rng = np.random.default_rng(7)
+    fTn = fT.detach().cpu().numpy()
+    for _ in range(5):
+        df = rng.standard_normal(shape)
+        dv = rng.standard_normal(3)
+        _, dfT = oracle.tlm(f0, df, np.array(v), dv, (1, 1, 1), dt, T)
+        lhs = float(np.sum((fTn - target) * dfT))
+        rhs = float(np.sum(gf * df) + np.dot(gv, dv))
+        assert abs(lhs - rhs) <= 1e-11 * abs(lhs)
+
+
+def test_fp32_adjoint_close_to_fp64():
+    shape, v, dt, T, f0, target = _setup(shape=(16, 16, 64), T=10, seed=17)
+    l64, g64, v64, _ = _tape_grad(f0, target, v, dt, T)
+    l32, g32, v32, _ = _tape_grad(f0, target, v, dt, T, f_dtype=torch.float32)
+    assert np.abs(g32 - g64).max() <= 1e-5 * np.abs(g64).max()
+    assert np.allclose(v32, v64, rtol=1e-4)
+
+
+def test_misfit_and_sum_are_compensated():
+    import math
+    from fidibench_amd.autograd import compensated_sum, misfit
+    g = torch.Generator().manual_seed(1)
+    x = (torch.randn(1 << 20, dtype=torch.float64, generator=g) * 1e6)
+    x[::7] += 1e15
+    x[3::7] -= 1e15
+    exact = math.fsum(x.numpy())
+    got = compensated_sum(x.cuda())
+    assert abs(got - exact) <= 2 * abs(exact) * 2 ** -52 + 1e-9
+    a = torch.randn(1000, dtype=torch.float64, generator=g)
+    b = torch.randn(1000, dtype=torch.float64, generator=g)
+    ref = 0.5 * math.fsum(((a - b) ** 2).numpy())
+    assert abs(float(misfit(a.cuda(), b.cuda())) - ref) <= 4 * ref * 2 ** -52

@@ -196,3 +196,18 @@ def test_advect_dev_matches_handle(oracle):
     torch.cuda.synchronize()
     out = (b if res.value else a).cpu().numpy()
     assert np.array_equal(bits(out), bits(ref))
+
+
+@pytest.mark.parametrize("name", ["vtk3d", "vtk2d", "vtk1d"])
+def test_save_vtk_handle_matches_reference_bytes(manifest, tmp_path, name):
+    """Upwind.saveVTK after advect, byte-compared with the reference's file."""
+    import os
+    from conftest import GOLDEN
+    m = manifest["vtk"][name]
+    f0, _ = load_case(name)
+    up = _upwind(m["shape"], m["velocity"], m["lengths"])
+    up.set_field(f0)
+    up.advect(m["steps"], float(m["dt"]))
+    out = tmp_path / "x.vtk"
+    up.saveVTK(str(out))
+    assert out.read_bytes() == open(os.path.join(GOLDEN, m["file"]), "rb").read()
