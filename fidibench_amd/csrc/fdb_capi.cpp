@@ -450,7 +450,7 @@ int fdb_adjoint_step_dev(int ndims, const uint64_t* num_cells, const double* vel
   if (f_t && !vgrad_acc) return fail(FDB_ERR_INVALID, "adjoint_step_dev: vgrad_acc is null");
   double* part = nullptr;
   if (f_t) {
-    const int64_t nb = fdb::adjoint_blocks(lam_dtype, g, lam_next, lam);
+    const int64_t nb = fdb::adjoint_blocks(lam_dtype, f_dtype, g, lam_next, lam, f_t);
     if (int rc = dev_scratch((size_t)nb * 6 * sizeof(double), &part)) return rc;
   }
   hipError_t e = fdb::launch_adjoint_step(lam_dtype, f_t ? f_dtype : lam_dtype, g, lam_next, lam,

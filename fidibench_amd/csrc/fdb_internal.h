@@ -16,7 +16,8 @@ void set_last_error(const std::string& msg);
 
 // Number of blocks the adjoint kernel launches for these buffers (sizes
 // its per-block partial sums).
-int64_t adjoint_blocks(int lam_dtype, const Grid3& g, const void* lam_next, void* lam);
+int64_t adjoint_blocks(int lam_dtype, int f_dtype, const Grid3& g, const void* lam_next,
+                       void* lam, const void* f_t);
 
 // Geometry of one advect call, in the padded 3-D form every kernel uses:
 // an NDIMS < 3 problem is placed on the trailing axes and the missing

@@ -23,14 +23,6 @@ namespace {
 
 // ------------------------------------------------------------ storage types
 
-__host__ __device__ inline uint16_t f32_to_bf16(float x) {
-  uint32_t b = __builtin_bit_cast(uint32_t, x);
-  if ((b & 0x7f800000u) == 0x7f800000u && (b & 0x007fffffu))
-    return (uint16_t)((b >> 16) | 0x0040u);  // quiet NaN
-  b += 0x7fffu + ((b >> 16) & 1u);           // round to nearest, ties to even
-  return (uint16_t)(b >> 16);
-}
-
 __host__ __device__ inline float bf16_to_f32(uint16_t h) {
   return __builtin_bit_cast(float, (uint32_t)h << 16);
 }
@@ -47,11 +39,24 @@ template <> struct St<float> {
   __device__ static M ld(float x) { return x; }
   __device__ static float st(M x) { return x; }
 };
+// bf16 rounding on the device is gfx950's v_cvt_pk_bf16_f32: round to
+// nearest, ties to even, the rule the host side (fdb_capi.cpp) and the
+// oracle implement in software.
 template <> struct St<uint16_t> {
   using M = float;
   __device__ static M ld(uint16_t x) { return bf16_to_f32(x); }
-  __device__ static uint16_t st(M x) { return f32_to_bf16(x); }
+  __device__ static uint16_t st(M x) { return __builtin_bit_cast(uint16_t, (__bf16)x); }
 };
+
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// Two floats to two bf16 in one 32-bit word (low half = first), one
+// v_cvt_pk_bf16_f32.
+__device__ inline uint32_t pack_bf16x2(float lo, float hi) {
+  const f32x2 v = {lo, hi};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
+}
 
 template <typename TS, int VEC>
 struct alignas(sizeof(TS) * VEC) Pack {
@@ -89,146 +94,6 @@ struct MarchArgs {
   int64_t chunk;            // planes per block along axis 0
   int64_t tiles2, tiles1;   // tiles along axis 2 and axis 1
 };
-
-// ------------------------------------------------------ one-step kernel --
-//
-// Block = 64 lanes along axis 2 (each lane VEC contiguous cells, one
-// 16-byte access) x WY rows along axis 1.  A block owns a tile of the
-// (axis1, axis2) plane and marches along axis 0 through `chunk` planes,
-// in the direction that makes the axis-0 neighbour the plane it loaded
-// one iteration before (kept in registers).  The axis-2 neighbour comes
-// from the adjacent lane (cross-lane shuffle) except at the tile edge,
-// the axis-1 neighbour is a load of the adjacent row (an L1/L2 hit: the
-// neighbouring wave loads that row in the same iteration).
-//
-// ADJ = true runs the transpose (directions already negated in args) and
-// also accumulates the velocity-gradient products f_t[i] * d_j(i), with
-// d_j(i) = lam_next[i + dir_j e_j] - lam_next[i], into per-block partial
-// sums (Neumaier + TwoProduct in FP64).
-template <typename TS, typename TF, int VEC, int WY, bool ADJ>
-__global__ __launch_bounds__(64 * WY) void march_kernel(const TS* __restrict__ in,
-                                                        TS* __restrict__ out, MarchArgs a,
-                                                        Coef<typename St<TS>::M> cf,
-                                                        const TF* __restrict__ ft,
-                                                        double* __restrict__ partials) {
-  using M = typename St<TS>::M;
-  const int lane = threadIdx.x;
-  const int row = threadIdx.y;
-
-  // XCD-aware tile order: consecutive hardware block ids go round-robin
-  // over the 8 XCDs; give each XCD a contiguous range of tiles so tiles
-  // that share halo rows and chunk-boundary planes share one L2.
-  int64_t b = blockIdx.x;
-  const int64_t nb = gridDim.x;
-  if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
-  const int64_t ti2 = b % a.tiles2;
-  b /= a.tiles2;
-  const int64_t ti1 = b % a.tiles1;
-  const int64_t ch = b / a.tiles1;
-
-  const int64_t i2 = (ti2 * 64 + lane) * VEC;
-  const int64_t i1 = ti1 * WY + row;
-  const bool valid = (i1 < a.n1) && (i2 < a.n2);
-  const int64_t i2c = valid ? i2 : 0;  // keeps idle lanes' loads in bounds
-  const int64_t i1c = valid ? i1 : 0;
-
-  const int64_t plane = a.n1 * a.n2;
-  const int64_t off_c = i1c * a.n2 + i2c;
-  const int64_t off_1 = wrap(i1c + a.dir1, a.n1) * a.n2 + i2c;
-  const bool lo_side = a.dir2 < 0;
-  const bool edge = lo_side ? (lane == 0 || i2c == 0) : (lane == 63 || i2c + VEC >= a.n2);
-  const int64_t off_e = i1c * a.n2 + (lo_side ? wrap(i2c - 1, a.n2) : wrap(i2c + VEC, a.n2));
-
-  const int64_t p_lo = ch * a.chunk;
-  const int64_t p_hi = (p_lo + a.chunk < a.n0) ? p_lo + a.chunk : a.n0;
-  const int64_t step = a.dir0 < 0 ? 1 : -1;
-  int64_t p = a.dir0 < 0 ? p_lo : p_hi - 1;
-
-  M prev[VEC];
-  load_pack<TS, VEC>(in + wrap(p + a.dir0, a.n0) * plane + off_c, prev);
-
-  double s[3] = {0.0, 0.0, 0.0}, e[3] = {0.0, 0.0, 0.0};
-
-  for (int64_t k = p_lo; k < p_hi; ++k, p += step) {
-    const TS* base = in + p * plane;
-    M c[VEC], n1[VEC], n2[VEC];
-    load_pack<TS, VEC>(base + off_c, c);
-    load_pack<TS, VEC>(base + off_1, n1);
-    M ev = M(0);
-    if (edge) ev = St<TS>::ld(base[off_e]);
-    if (lo_side) {
-      const M sh = __shfl_up(c[VEC - 1], 1);
-      n2[0] = edge ? ev : sh;
-#pragma unroll
-      for (int v = 1; v < VEC; ++v) n2[v] = c[v - 1];
-    } else {
-      const M sh = __shfl_down(c[0], 1);
-      n2[VEC - 1] = edge ? ev : sh;
-#pragma unroll
-      for (int v = 0; v < VEC - 1; ++v) n2[v] = c[v + 1];
-    }
-    M o[VEC];
-#pragma unroll
-    for (int v = 0; v < VEC; ++v) {
-      const M x = c[v];
-      M g = x;
-      const M d0 = prev[v] - x;
-      const M d1 = n1[v] - x;
-      const M d2 = n2[v] - x;
-      if (a.act0) g = g - cf.c[0] * d0;
-      if (a.act1) g = g - cf.c[1] * d1;
-      if (a.act2) g = g - cf.c[2] * d2;
-      o[v] = g;
-      if constexpr (ADJ) {
-        if (ft != nullptr && valid) {
-          const double fv = (double)St<TF>::ld(ft[p * plane + off_c + v]);
-          const double dd[3] = {(double)prev[v] - (double)x, (double)n1[v] - (double)x,
-                                (double)n2[v] - (double)x};
-#pragma unroll
-          for (int j = 0; j < 3; ++j) {
-            const double pr = fv * dd[j];
-            const double pe = __builtin_fma(fv, dd[j], -pr);  // exact product error
-            const double t = s[j] + pr;
-            e[j] += (__builtin_fabs(s[j]) >= __builtin_fabs(pr)) ? (s[j] - t) + pr : (pr - t) + s[j];
-            e[j] += pe;
-            s[j] = t;
-          }
-        }
-      }
-      prev[v] = x;
-    }
-    if (valid) store_pack<TS, VEC>(out + p * plane + off_c, o);
-  }
-
-  if constexpr (ADJ) {
-    if (ft != nullptr) {
-      // Block reduction of three {sum, comp} pairs through LDS.
-      __shared__ double red[64 * WY][6];
-      const int t = row * 64 + lane;
-#pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        red[t][2 * j] = s[j];
-        red[t][2 * j + 1] = e[j];
-      }
-      __syncthreads();
-      for (int w = 32 * WY; w > 0; w >>= 1) {
-        if (t < w) {
-#pragma unroll
-          for (int j = 0; j < 3; ++j) {
-            const double s1 = red[t][2 * j], s2 = red[t + w][2 * j];
-            const double sum = s1 + s2;
-            const double bb = sum - s1;
-            const double err = (s1 - (sum - bb)) + (s2 - bb);  // TwoSum
-            red[t][2 * j] = sum;
-            red[t][2 * j + 1] = red[t][2 * j + 1] + red[t + w][2 * j + 1] + err;
-          }
-        }
-        __syncthreads();
-      }
-      if (t < 6) partials[blockIdx.x * 6 + t] = red[0][t];
-    }
-  }
-}
 
 // Adds npart per-block {sum, comp} x3 partials into acc (device, 6
 // doubles) in a fixed order: one block, deterministic.
@@ -500,8 +365,15 @@ template <> struct RawT<16> { using T = uint32_t __attribute__((ext_vector_type(
 template <typename TS, int VEC, bool NT>
 __device__ inline void store_pack_nt(TS* p, const typename St<TS>::M (&in)[VEC]) {
   Pack<TS, VEC> pk;
+  if constexpr (sizeof(TS) == 2 && VEC % 2 == 0) {
+    uint32_t w[VEC / 2];
 #pragma unroll
-  for (int v = 0; v < VEC; ++v) pk.v[v] = St<TS>::st(in[v]);
+    for (int v = 0; v < VEC / 2; ++v) w[v] = pack_bf16x2(in[2 * v], in[2 * v + 1]);
+    pk = __builtin_bit_cast(Pack<TS, VEC>, w);
+  } else {
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) pk.v[v] = St<TS>::st(in[v]);
+  }
   using R = typename RawT<sizeof(Pack<TS, VEC>)>::T;
   const R raw = __builtin_bit_cast(R, pk);
   if constexpr (NT)
@@ -510,11 +382,27 @@ __device__ inline void store_pack_nt(TS* p, const typename St<TS>::M (&in)[VEC])
     *reinterpret_cast<R*>(p) = raw;
 }
 
-template <typename TS, int VEC, int R, int WY, bool NT>
+// Per-thread compensated accumulators of the three velocity-gradient dot
+// products (adjoint kernel only).
+struct VAcc {
+  double s[3] = {0.0, 0.0, 0.0}, e[3] = {0.0, 0.0, 0.0};
+  __device__ void add(int j, double x) {  // Neumaier
+    const double t = s[j] + x;
+    e[j] += (__builtin_fabs(s[j]) >= __builtin_fabs(x)) ? (s[j] - t) + x : (x - t) + s[j];
+    s[j] = t;
+  }
+};
+
+// ADJ = true: the transpose step (a.dir* = -u) plus, when ft != nullptr,
+// the products f_t[i] * (lam_next[i + dir_j e_j] - lam_next[i]) summed per
+// plane in FP64 and added to acc (sum_i lam_next[i] (f_t[up_j(i)] - f_t[i])
+// by the periodic shift i -> i + dir_j e_j).
+template <typename TS, typename TF, int VEC, int R, int WY, bool NT, bool ADJ>
 __device__ inline void rows_tile(const TS* __restrict__ in, TS* __restrict__ out,
                                  const MarchArgs& a, const Coef<typename St<TS>::M>& cf,
-                                 int64_t b) {
+                                 int64_t b, const TF* __restrict__ ft, VAcc& acc) {
   using M = typename St<TS>::M;
+  const bool with_f = ADJ && ft != nullptr;
   const int lane = threadIdx.x;
   const int wave = threadIdx.y;
 
@@ -563,7 +451,9 @@ __device__ inline void rows_tile(const TS* __restrict__ in, TS* __restrict__ out
 #pragma unroll
     for (int r = 0; r < R; ++r) load_pack<TS, VEC>(base + roff[r], prev[r]);
   }
+  using MF = typename St<TF>::M;
   M cur[R][VEC], hal[VEC], ed[R];
+  MF fc[ADJ ? R : 1][VEC];
   auto load_plane = [&](int64_t pp, M (&c)[R][VEC], M (&h)[VEC], M (&e)[R]) {
     const TS* base = in + pp * plane;
 #pragma unroll
@@ -572,11 +462,26 @@ __device__ inline void rows_tile(const TS* __restrict__ in, TS* __restrict__ out
 #pragma unroll
     for (int r = 0; r < R; ++r) e[r] = edge ? St<TS>::ld(base[roff[r] + i2e]) : M(0);
   };
+  auto load_f = [&](int64_t pp, MF (&f)[ADJ ? R : 1][VEC]) {
+    if constexpr (ADJ) {
+      if (with_f) {
+        const TF* fb = ft + pp * plane + i2c;
+#pragma unroll
+        for (int r = 0; r < R; ++r) load_pack<TF, VEC>(fb + roff[r], f[r]);
+      }
+    }
+  };
   load_plane(p, cur, hal, ed);
+  load_f(p, fc);
 
   for (int64_t k = 0; k < cnt; ++k) {
     M nxt[R][VEC], nhal[VEC], ned[R];
-    if (k + 1 < cnt) load_plane(p + step, nxt, nhal, ned);
+    MF fn[ADJ ? R : 1][VEC];
+    if (k + 1 < cnt) {
+      load_plane(p + step, nxt, nhal, ned);
+      load_f(p + step, fn);
+    }
+    double ps[3] = {0.0, 0.0, 0.0};
     // axis-2 neighbours: shuffles are done by all lanes together
     M n2[R][VEC];
 #pragma unroll
@@ -610,9 +515,23 @@ __device__ inline void rows_tile(const TS* __restrict__ in, TS* __restrict__ out
         if (a.act1) g = g - cf.c[1] * (n1 - x);
         if (a.act2) g = g - cf.c[2] * (n2[r][v] - x);
         o[v] = g;
+        if constexpr (ADJ) {
+          if (with_f && col_ok && r < nvalid) {
+            const double xd = (double)x, fv = (double)fc[r][v];
+            ps[0] += fv * ((double)prev[r][v] - xd);
+            ps[1] += fv * ((double)n1 - xd);
+            ps[2] += fv * ((double)n2[r][v] - xd);
+          }
+        }
         prev[r][v] = x;
       }
       if (col_ok && r < nvalid) store_pack_nt<TS, VEC, NT>(obase + roff[r], o);
+    }
+    if constexpr (ADJ) {
+      if (with_f) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) acc.add(j, ps[j]);
+      }
     }
     if (k + 1 < cnt) {
 #pragma unroll
@@ -620,6 +539,10 @@ __device__ inline void rows_tile(const TS* __restrict__ in, TS* __restrict__ out
 #pragma unroll
         for (int v = 0; v < VEC; ++v) cur[r][v] = nxt[r][v];
         ed[r] = ned[r];
+        if constexpr (ADJ) {
+#pragma unroll
+          for (int v = 0; v < VEC; ++v) fc[r][v] = fn[r][v];
+        }
       }
 #pragma unroll
       for (int v = 0; v < VEC; ++v) hal[v] = nhal[v];
@@ -628,26 +551,60 @@ __device__ inline void rows_tile(const TS* __restrict__ in, TS* __restrict__ out
   }
 }
 
+// Block reduction of the per-thread accumulators into partials[block*6].
+template <int NT_>
+__device__ inline void block_reduce_vacc(const VAcc& acc, double* __restrict__ partials) {
+  __shared__ double red[NT_][6];
+  const int t = threadIdx.y * 64 + threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    red[t][2 * j] = acc.s[j];
+    red[t][2 * j + 1] = acc.e[j];
+  }
+  __syncthreads();
+  for (int w = NT_ / 2; w > 0; w >>= 1) {
+    if (t < w) {
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double s1 = red[t][2 * j], s2 = red[t + w][2 * j];
+        const double sum = s1 + s2;
+        const double bb = sum - s1;
+        red[t][2 * j + 1] =
+            red[t][2 * j + 1] + red[t + w][2 * j + 1] + ((s1 - (sum - bb)) + (s2 - bb));
+        red[t][2 * j] = sum;
+      }
+    }
+    __syncthreads();
+  }
+  if (t < 6) partials[blockIdx.x * 6 + t] = red[0][t];
+}
+
 // Tiles are numbered axis-2 fastest, then axis 1, then chunk of axis 0.
 // Hardware hands blocks to the 8 XCDs round-robin by block id; each XCD
 // gets a contiguous range of tile numbers so tiles that share halo rows
 // and chunk-boundary planes share that XCD's L2.  With gridDim.x < ntiles
 // the grid is persistent: block (xcd, l) takes tiles start+l, start+l+n8..
-template <typename TS, int VEC, int R, int WY, bool NT>
+template <typename TS, typename TF, int VEC, int R, int WY, bool NT, bool ADJ>
 __global__ __launch_bounds__(64 * WY) void rows_kernel(const TS* __restrict__ in,
                                                        TS* __restrict__ out, MarchArgs a,
                                                        Coef<typename St<TS>::M> cf,
-                                                       int64_t ntiles) {
+                                                       int64_t ntiles, const TF* __restrict__ ft,
+                                                       double* __restrict__ partials) {
   const int64_t nb = gridDim.x;
+  VAcc acc;
   if ((nb & 7) == 0) {
     const int64_t xcd = blockIdx.x & 7, l = blockIdx.x >> 3, n8 = nb >> 3;
     const int64_t per = (ntiles + 7) >> 3;
     const int64_t lo = xcd * per;
     const int64_t hi = lo + per < ntiles ? lo + per : ntiles;
-    for (int64_t t = lo + l; t < hi; t += n8) rows_tile<TS, VEC, R, WY, NT>(in, out, a, cf, t);
+    for (int64_t t = lo + l; t < hi; t += n8)
+      rows_tile<TS, TF, VEC, R, WY, NT, ADJ>(in, out, a, cf, t, ft, acc);
   } else {
     for (int64_t t = blockIdx.x; t < ntiles; t += nb)
-      rows_tile<TS, VEC, R, WY, NT>(in, out, a, cf, t);
+      rows_tile<TS, TF, VEC, R, WY, NT, ADJ>(in, out, a, cf, t, ft, acc);
+  }
+  if constexpr (ADJ) {
+    if (ft != nullptr) block_reduce_vacc<64 * WY>(acc, partials);
   }
 }
 
@@ -656,95 +613,68 @@ __global__ __launch_bounds__(64 * WY) void rows_kernel(const TS* __restrict__ in
 template <typename TS>
 constexpr int max_vec() { return (int)(16 / sizeof(TS)); }
 
-// Launch shape of the streaming forward kernel.  The default comes from
-// sweeps at 512^3 on MI355X; FDB_ROWS_CFG="R,WY,CHUNK,NT[,GRID]" overrides
-// it (R and WY from the instantiated set below; GRID 0 = one block per
-// tile, else a persistent grid of that many blocks).
+// Launch shape of the rows kernel.  Defaults come from sweeps at 512^3 on
+// MI355X (see RowsCfg); FDB_ROWS_CFG="R,WY,CHUNK,NT[,GRID]" overrides
+// every storage type, FDB_ROWS_CFG_<bytes> one of them (_8 FP64, _4 FP32,
+// _2 BF16) and FDB_ADJ_CFG_<bytes> the adjoint.  CHUNK 0 = automatic,
+// GRID 0 = one block per tile.
 struct RowsCfg {
-  // 512^3 FP64 sweep: R=8 rows x 2 waves, 64-plane chunks, streaming
-  // stores and a persistent grid of 1536 blocks (6 per CU) run at the
+  // FP64 forward: R=8 rows x 2 waves, ~1024 equal tiles (64-plane chunks
+  // at 512^3, one tile per block) and streaming stores run at the
   // plain-copy bandwidth of the device; fewer rows per thread or more
-  // waves per block lose 2-8 %, R=16 loses 25 % (one wave per SIMD).
-  int R = 8, WY = 2, chunk = 64, nt = 1, grid = 1536;
+  // waves per block lose 2-8 %, R=16 loses 25 % (one wave per SIMD), and
+  // grids that leave some blocks a second tile lose up to a third.
+  int R = 8, WY = 2, chunk = 0, nt = 1, grid = 0;
 };
 
-RowsCfg rows_cfg() {
-  static RowsCfg cfg = [] {
-    RowsCfg c;
-    if (const char* s = getenv("FDB_ROWS_CFG")) {
-      int r, w, ch, nt, gr = 0;
-      if (sscanf(s, "%d,%d,%d,%d,%d", &r, &w, &ch, &nt, &gr) >= 4) {
-        c.R = r;
-        c.WY = w;
-        c.chunk = ch;
-        c.nt = nt;
-        c.grid = gr;
-      }
-    }
-    return c;
-  }();
-  return cfg;
-}
+constexpr int64_t kTargetBlocks = 1024;
 
-template <typename TS>
-hipError_t rows_dispatch(const Grid3& g, const TS* in, TS* out, hipStream_t s) {
-  using M = typename St<TS>::M;
-  const RowsCfg cfg = rows_cfg();
-  MarchArgs a;
-  a.n0 = g.n[0];
-  a.n1 = g.n[1];
-  a.n2 = g.n[2];
-  a.dir0 = g.up[0];
-  a.dir1 = g.up[1];
-  a.dir2 = g.up[2];
-  a.act0 = g.act[0];
-  a.act1 = g.act[1];
-  a.act2 = g.act[2];
-  Coef<M> cf;
-  for (int j = 0; j < 3; ++j) cf.c[j] = (M)g.c[j];
-  int vec = max_vec<TS>();
-  const uintptr_t align_bits = (uintptr_t)in | (uintptr_t)out;
-  while (vec > 1 && ((a.n2 % vec) != 0 || (align_bits % (vec * sizeof(TS))) != 0)) vec >>= 1;
-  const int R = cfg.R, WY = cfg.WY;
-  a.tiles2 = (a.n2 + 64 * vec - 1) / (64 * vec);
-  a.tiles1 = (a.n1 + (int64_t)WY * R - 1) / ((int64_t)WY * R);
-  const int64_t tiles = a.tiles2 * a.tiles1;
-  int64_t chunk = cfg.chunk > 0 ? cfg.chunk : 32;
-  while (chunk > 1 && tiles * ((a.n0 + chunk - 1) / chunk) < 2048) chunk >>= 1;
-  a.chunk = chunk;
-  const int64_t ntiles = tiles * ((a.n0 + chunk - 1) / chunk);
-  const int64_t nb = (cfg.grid > 0 && cfg.grid < ntiles) ? cfg.grid : ntiles;
-  if (nb <= 0 || nb > 0x7fffffffLL) return hipErrorInvalidConfiguration;
-  bool launched = false;
-#define FDB_ROWS(V, RR, W, NTF)                                                              \
-  if constexpr (V <= max_vec<TS>()) {                                                        \
-    if (!launched && vec == V && R == RR && WY == W && (cfg.nt != 0) == NTF) {              \
-      hipLaunchKernelGGL((rows_kernel<TS, V, RR, W, NTF>), dim3((unsigned)nb), dim3(64, W), \
-                         0, s, in, out, a, cf, ntiles);                                      \
-      launched = true;                                                                       \
-    }                                                                                        \
+RowsCfg parse_cfg(const char* s, RowsCfg c) {
+  int r, w, ch, nt, gr = 0;
+  if (s && sscanf(s, "%d,%d,%d,%d,%d", &r, &w, &ch, &nt, &gr) >= 4) {
+    c.R = r;
+    c.WY = w;
+    c.chunk = ch;
+    c.nt = nt;
+    c.grid = gr;
   }
-#define FDB_ROWS_V(RR, W, NTF) \
-  FDB_ROWS(8, RR, W, NTF) FDB_ROWS(4, RR, W, NTF) FDB_ROWS(2, RR, W, NTF) FDB_ROWS(1, RR, W, NTF)
-  FDB_ROWS_V(8, 2, true)
-  FDB_ROWS_V(8, 1, true)
-  FDB_ROWS_V(8, 4, true)
-  FDB_ROWS_V(16, 1, true)
-  FDB_ROWS_V(16, 2, true)
-  FDB_ROWS_V(4, 4, true)
-  FDB_ROWS_V(8, 2, false)
-#undef FDB_ROWS_V
-#undef FDB_ROWS
-  if (!launched) return hipErrorInvalidValue;
-  return hipGetLastError();
+  return c;
 }
 
+RowsCfg rows_cfg(int bytes, bool adj) {
+  static RowsCfg cfgs[2][9];
+  static bool init[2][9] = {};
+  if (!init[adj][bytes]) {
+    RowsCfg c;
+    // 512^3 sweeps: FP32 R=4 x 2 waves (715 G cells/s, 2.0x FP64), BF16
+    // R=2 x 4 waves; the adjoint carries f_t as well: R=8 x 1 wave.
+    if (adj) {
+      c.R = 8;
+      c.WY = 1;
+    } else if (bytes == 4) {
+      c.R = 4;
+    } else if (bytes == 2) {
+      c.R = 2;
+      c.WY = 4;
+    }
+    char name[32];
+    snprintf(name, sizeof(name), adj ? "FDB_ADJ_CFG_%d" : "FDB_ROWS_CFG_%d", bytes);
+    const char* s = getenv(name);
+    if (!s && !adj) s = getenv("FDB_ROWS_CFG");
+    cfgs[adj][bytes] = parse_cfg(s, c);
+    init[adj][bytes] = true;
+  }
+  return cfgs[adj][bytes];
+}
+
+// Launches the rows kernel for one step (forward: dirs = u, adjoint:
+// dirs = -u).  With dry = true only reports the block count.
 template <typename TS, typename TF, bool ADJ>
-hipError_t march_dispatch(const Grid3& g, const TS* in, TS* out, const int dirs[3],
-                          const TF* ft, double* partials, int64_t* nblocks_out,
-                          hipStream_t s, bool dry) {
+hipError_t rows_dispatch(const Grid3& g, const TS* in, TS* out, const int dirs[3],
+                         const TF* ft, double* partials, int64_t* nblocks_out, hipStream_t s,
+                         bool dry) {
   using M = typename St<TS>::M;
-  constexpr int WY = 4;
+  const RowsCfg cfg = rows_cfg((int)sizeof(TS), ADJ);
   MarchArgs a;
   a.n0 = g.n[0];
   a.n1 = g.n[1];
@@ -757,49 +687,78 @@ hipError_t march_dispatch(const Grid3& g, const TS* in, TS* out, const int dirs[
   a.act2 = g.act[2];
   Coef<M> cf;
   for (int j = 0; j < 3; ++j) cf.c[j] = (M)g.c[j];
-
-  // Widest access that keeps every row start aligned.
+  // Widest access that keeps every row start aligned (f_t included).
   int vec = max_vec<TS>();
-  const uintptr_t align_bits = (uintptr_t)in | (uintptr_t)out;
-  while (vec > 1 && ((a.n2 % vec) != 0 || (align_bits % (vec * sizeof(TS))) != 0)) vec >>= 1;
+  uintptr_t align_bits = (uintptr_t)in | (uintptr_t)out;
+  while (vec > 1 && ((a.n2 % vec) != 0 || (align_bits % (vec * sizeof(TS))) != 0 ||
+                     ((uintptr_t)ft % (vec * sizeof(TF))) != 0))
+    vec >>= 1;
+  const int R = cfg.R, WY = cfg.WY;
   a.tiles2 = (a.n2 + 64 * vec - 1) / (64 * vec);
-  a.tiles1 = (a.n1 + WY - 1) / WY;
-  // Planes per block: enough blocks to fill 256 CUs several times over.
-  int64_t tiles = a.tiles2 * a.tiles1;
-  int64_t chunk = 16;
-  while (chunk > 1 && tiles * ((a.n0 + chunk - 1) / chunk) < 2048) chunk >>= 1;
+  a.tiles1 = (a.n1 + (int64_t)WY * R - 1) / ((int64_t)WY * R);
+  const int64_t tiles = a.tiles2 * a.tiles1;
+  // Axis-0 chunk: an explicit one, or enough equal chunks for about
+  // kTargetBlocks tiles (4 per CU), each block taking one tile.
+  int64_t chunk = cfg.chunk;
+  if (chunk <= 0) {
+    const int64_t nch = std::max<int64_t>(1, (kTargetBlocks + tiles / 2) / tiles);
+    chunk = (a.n0 + nch - 1) / nch;
+  }
   a.chunk = chunk;
-  const int64_t nchunks = (a.n0 + chunk - 1) / chunk;
-  const int64_t nb = tiles * nchunks;
+  const int64_t ntiles = tiles * ((a.n0 + chunk - 1) / chunk);
+  const int64_t nb = (cfg.grid > 0 && cfg.grid < ntiles) ? cfg.grid : ntiles;
   if (nblocks_out) *nblocks_out = nb;
   if (dry) return hipSuccess;
   if (nb <= 0 || nb > 0x7fffffffLL) return hipErrorInvalidConfiguration;
-  const dim3 block(64, WY);
-  switch (vec) {
-#define FDB_LAUNCH(V)                                                                        \
-  case V:                                                                                    \
-    if constexpr (V <= max_vec<TS>()) {                                                      \
-      hipLaunchKernelGGL((march_kernel<TS, TF, V, WY, ADJ>), dim3((unsigned)nb), block, 0, s, \
-                         in, out, a, cf, ft, partials);                                      \
-    }                                                                                        \
-    break;
-    FDB_LAUNCH(8)
-    FDB_LAUNCH(4)
-    FDB_LAUNCH(2)
-    FDB_LAUNCH(1)
-#undef FDB_LAUNCH
-    default:
-      return hipErrorInvalidValue;
+  bool launched = false;
+#define FDB_ROWS(V, RR, W, NTF)                                                             \
+  if constexpr (V <= max_vec<TS>() && V <= max_vec<TF>()) {                                 \
+    if (!launched && vec == V && R == RR && WY == W && (cfg.nt != 0) == NTF) {             \
+      hipLaunchKernelGGL((rows_kernel<TS, TF, V, RR, W, NTF, ADJ>), dim3((unsigned)nb),     \
+                         dim3(64, W), 0, s, in, out, a, cf, ntiles, ft, partials);          \
+      launched = true;                                                                      \
+    }                                                                                       \
   }
+#define FDB_ROWS_V(RR, W, NTF) \
+  FDB_ROWS(8, RR, W, NTF) FDB_ROWS(4, RR, W, NTF) FDB_ROWS(2, RR, W, NTF) FDB_ROWS(1, RR, W, NTF)
+  FDB_ROWS_V(8, 2, true)
+  FDB_ROWS_V(4, 2, true)
+  FDB_ROWS_V(4, 4, true)
+  FDB_ROWS_V(8, 1, true)
+  FDB_ROWS_V(2, 4, true)
+  FDB_ROWS_V(2, 8, true)
+  FDB_ROWS_V(2, 2, true)
+  FDB_ROWS_V(1, 8, true)
+#undef FDB_ROWS_V
+#undef FDB_ROWS
+  if (!launched) return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
 template <typename TS>
 hipError_t step_typed(const Grid3& g, const void* in, void* out, hipStream_t s) {
-  if (getenv("FDB_MARCH_V1"))  // the first one-row-per-thread kernel, for comparison
-    return march_dispatch<TS, TS, false>(g, (const TS*)in, (TS*)out, g.up, (const TS*)nullptr,
-                                         nullptr, nullptr, s, false);
-  return rows_dispatch<TS>(g, (const TS*)in, (TS*)out, s);
+  return rows_dispatch<TS, TS, false>(g, (const TS*)in, (TS*)out, g.up, (const TS*)nullptr,
+                                      nullptr, nullptr, s, false);
+}
+
+// The transpose step: directions -u; with f_t the velocity-gradient
+// products too (lam in FP64 or FP32, f_t in any storage type).
+hipError_t adjoint_dispatch(int lam_dtype, int f_dtype, const Grid3& g, const void* lam_next,
+                            void* lam, const void* f_t, double* partials, int64_t* nb,
+                            hipStream_t s, bool dry) {
+  const int dirs[3] = {-g.up[0], -g.up[1], -g.up[2]};
+  if (f_t == nullptr) f_dtype = lam_dtype;
+#define FDB_ADJ(TL, TF)                                                                    \
+  return rows_dispatch<TL, TF, true>(g, (const TL*)lam_next, (TL*)lam, dirs, (const TF*)f_t, \
+                                     partials, nb, s, dry)
+  if (lam_dtype == FDB_F64 && f_dtype == FDB_F64) FDB_ADJ(double, double);
+  if (lam_dtype == FDB_F64 && f_dtype == FDB_F32) FDB_ADJ(double, float);
+  if (lam_dtype == FDB_F64 && f_dtype == FDB_BF16) FDB_ADJ(double, uint16_t);
+  if (lam_dtype == FDB_F32 && f_dtype == FDB_F64) FDB_ADJ(float, double);
+  if (lam_dtype == FDB_F32 && f_dtype == FDB_F32) FDB_ADJ(float, float);
+  if (lam_dtype == FDB_F32 && f_dtype == FDB_BF16) FDB_ADJ(float, uint16_t);
+#undef FDB_ADJ
+  return hipErrorInvalidValue;
 }
 
 }  // namespace
@@ -865,34 +824,18 @@ hipError_t launch_advect_pass(int dtype, const Grid3& g, void* const bufs[2], in
 hipError_t launch_adjoint_step(int lam_dtype, int f_dtype, const Grid3& g, const void* lam_next,
                                void* lam, const void* f_t, double* partials, double* acc,
                                hipStream_t s) {
-  const int dirs[3] = {-g.up[0], -g.up[1], -g.up[2]};
   int64_t nb = 0;
-  hipError_t err;
-#define FDB_ADJ(TL, TF)                                                                      \
-  err = march_dispatch<TL, TF, true>(g, (const TL*)lam_next, (TL*)lam, dirs, (const TF*)f_t, \
-                                     partials, &nb, s, false)
-  if (lam_dtype == FDB_F64 && f_dtype == FDB_F64) FDB_ADJ(double, double);
-  else if (lam_dtype == FDB_F64 && f_dtype == FDB_F32) FDB_ADJ(double, float);
-  else if (lam_dtype == FDB_F64 && f_dtype == FDB_BF16) FDB_ADJ(double, uint16_t);
-  else if (lam_dtype == FDB_F32 && f_dtype == FDB_F64) FDB_ADJ(float, double);
-  else if (lam_dtype == FDB_F32 && f_dtype == FDB_F32) FDB_ADJ(float, float);
-  else if (lam_dtype == FDB_F32 && f_dtype == FDB_BF16) FDB_ADJ(float, uint16_t);
-  else return hipErrorInvalidValue;
-#undef FDB_ADJ
+  hipError_t err = adjoint_dispatch(lam_dtype, f_dtype, g, lam_next, lam, f_t, partials, &nb, s,
+                                    false);
   if (err != hipSuccess || f_t == nullptr) return err;
   hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, partials, nb, 3, acc, 1);
   return hipGetLastError();
 }
 
-int64_t adjoint_blocks(int lam_dtype, const Grid3& g, const void* lam_next, void* lam) {
-  const int dirs[3] = {-g.up[0], -g.up[1], -g.up[2]};
+int64_t adjoint_blocks(int lam_dtype, int f_dtype, const Grid3& g, const void* lam_next,
+                       void* lam, const void* f_t) {
   int64_t nb = 0;
-  if (lam_dtype == FDB_F64)
-    (void)march_dispatch<double, double, true>(g, (const double*)lam_next, (double*)lam, dirs,
-                                               nullptr, nullptr, &nb, nullptr, true);
-  else if (lam_dtype == FDB_F32)
-    (void)march_dispatch<float, float, true>(g, (const float*)lam_next, (float*)lam, dirs,
-                                             nullptr, nullptr, &nb, nullptr, true);
+  (void)adjoint_dispatch(lam_dtype, f_dtype, g, lam_next, lam, f_t, nullptr, &nb, nullptr, true);
   return nb;
 }
 

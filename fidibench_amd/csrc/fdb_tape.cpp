@@ -11,6 +11,12 @@
 //     m = max(1, C(s+r-2, s), n - C(s-1+r, s-1))
 // and the two parts are reversed recursively; the total number of forward
 // steps is then r*n - C(s+r, s+1), the optimum.
+//
+// Memory traffic: the working state is a pointer.  A step reads it and
+// writes a scratch buffer -- or, when the schedule stores the state it
+// produces, the snapshot slot itself -- so storing and restoring cost no
+// copies; with enough memory for every state the forward pass moves the
+// same bytes as a plain advect.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -96,9 +102,6 @@ int tfail(int code, const std::string& m) {
 
 struct fdb_tape {
   int ndims = 0;
-  uint64_t n[3] = {1, 1, 1};
-  double v[3] = {0, 0, 0}, len[3] = {0, 0, 0};
-  double dt = 0.0;
   int nsteps = 0;
   int f_dtype = FDB_F64, lam_dtype = FDB_F64;
   fdb::Grid3 g{};
@@ -107,13 +110,13 @@ struct fdb_tape {
   Plan plan;
   size_t resume = 0;  // first plan action left for backward
   // device memory
-  void* base = nullptr;            // state 0
-  std::vector<void*> slots;        // snapshot stack storage
-  std::vector<int> stack_steps;    // steps of the stored states (top = back)
-  void* work[2] = {nullptr, nullptr};
-  int wcur = 0;
-  int wstep = -1;                  // state held in work[wcur]
-  int time_block = fdb::kMaxTimeBlock;  // steps per forward pass
+  void* base = nullptr;          // state 0 (a copy of f0)
+  std::vector<void*> slots;      // snapshot storage, used as a stack
+  std::vector<int> stack_steps;  // steps of the stored states (top = back)
+  void* scratch[2] = {nullptr, nullptr};
+  const void* w = nullptr;       // where the working state lives
+  int wstep = -1;                // which state it is
+  int time_block = fdb::kMaxTimeBlock;
   void* lam_tmp = nullptr;
   double* partials = nullptr;
   size_t partials_bytes = 0;
@@ -139,7 +142,7 @@ void tape_free(fdb_tape* t) {
   (void)hipDeviceSynchronize();
   if (t->base) (void)hipFree(t->base);
   for (void* s : t->slots) (void)hipFree(s);
-  for (void* w : t->work)
+  for (void* w : t->scratch)
     if (w) (void)hipFree(w);
   if (t->lam_tmp) (void)hipFree(t->lam_tmp);
   if (t->partials) (void)hipFree(t->partials);
@@ -147,21 +150,100 @@ void tape_free(fdb_tape* t) {
   delete t;
 }
 
-// Advance the working state by a steps.
-int advance(fdb_tape* t, int a, hipStream_t s) {
-  int left = a;
-  while (left > 0) {
-    const int k = std::min(left, t->time_block);
-    TAPE_HIP(fdb::launch_advect_pass(t->f_dtype, t->g, t->work, &t->wcur, k, s, nullptr));
-    left -= k;
-  }
-  t->wstep += a;
-  t->fwd_steps += a;
+// One step from the working state into dst (dst != working state).
+int step_into(fdb_tape* t, void* dst, hipStream_t s) {
+  void* const pair[2] = {const_cast<void*>(t->w), dst};
+  int cur = 0;
+  TAPE_HIP(fdb::launch_advect_pass(t->f_dtype, t->g, pair, &cur, 1, s, nullptr));
+  t->w = dst;
+  t->wstep += 1;
+  t->fwd_steps += 1;
   return FDB_OK;
 }
 
-void* top_buffer(fdb_tape* t) {
+// A scratch buffer that is not the working state.
+void* free_scratch(fdb_tape* t) { return t->w == t->scratch[0] ? t->scratch[1] : t->scratch[0]; }
+
+// Advance the working state by a steps; the last one lands in `last_dst`
+// when given (the slot of a snapshot about to be stored).
+int advance(fdb_tape* t, int a, void* last_dst, hipStream_t s) {
+  for (int i = 0; i < a; ++i) {
+    void* dst = (i == a - 1 && last_dst) ? last_dst : free_scratch(t);
+    if (int rc = step_into(t, dst, s)) return rc;
+  }
+  return FDB_OK;
+}
+
+const void* top_buffer(const fdb_tape* t) {
   return t->stack_steps.size() == 1 ? t->base : t->slots[t->stack_steps.size() - 2];
+}
+
+// The slot the next STORE fills.
+void* next_slot(fdb_tape* t) { return t->slots[t->stack_steps.size() - 1]; }
+
+// Executes plan actions from index i; in forward mode stops at the first
+// RESTORE or ADJOINT and returns its index through *stop.
+int run_plan(fdb_tape* t, size_t i, bool forward_only, hipStream_t s, const void** lam_next,
+             void* lam0, int* done, size_t* stop) {
+  const auto& acts = t->plan.acts;
+  for (; i < acts.size(); ++i) {
+    const auto& a = acts[i];
+    switch (a.first) {
+      case FDB_CK_ADVANCE: {
+        const bool store_next = i + 1 < acts.size() && acts[i + 1].first == FDB_CK_STORE;
+        if (int rc = advance(t, a.second, store_next ? next_slot(t) : nullptr, s)) return rc;
+        break;
+      }
+      case FDB_CK_STORE:
+        if (t->w != next_slot(t)) {  // the state was not produced in place
+          TAPE_HIP(hipMemcpyAsync(next_slot(t), t->w, t->fbytes, hipMemcpyDeviceToDevice, s));
+          t->w = next_slot(t);
+        }
+        t->stack_steps.push_back(t->wstep);
+        break;
+      case FDB_CK_POP:
+        t->stack_steps.pop_back();
+        break;
+      case FDB_CK_RESTORE: {
+        if (forward_only) {
+          *stop = i;
+          return FDB_OK;
+        }
+        // Skip the restore and the advance after it when the working
+        // state is already the state they would produce.
+        const int top = t->stack_steps.back();
+        const bool adv = i + 1 < acts.size() && acts[i + 1].first == FDB_CK_ADVANCE;
+        const int target = top + (adv ? acts[i + 1].second : 0);
+        if (t->wstep == target) {
+          if (adv) ++i;
+          break;
+        }
+        t->w = top_buffer(t);
+        t->wstep = top;
+        break;
+      }
+      case FDB_CK_ADJOINT: {
+        if (forward_only) {
+          *stop = i;
+          return FDB_OK;
+        }
+        if (t->wstep != a.second)
+          return tfail(FDB_ERR_INVALID, "tape_backward: schedule out of step (internal)");
+        ++*done;
+        // The last adjoint step writes into lam0.
+        void* dst = ((t->nsteps - *done) % 2 == 0) ? lam0 : t->lam_tmp;
+        TAPE_HIP(fdb::launch_adjoint_step(t->lam_dtype, t->f_dtype, t->g, *lam_next, dst, t->w,
+                                          t->partials, t->acc, s));
+        *lam_next = dst;
+        ++t->adj_steps;
+        break;
+      }
+      default:
+        return tfail(FDB_ERR_INVALID, "tape: bad action (internal)");
+    }
+  }
+  if (stop) *stop = acts.size();
+  return FDB_OK;
 }
 
 }  // namespace
@@ -197,19 +279,13 @@ int fdb_tape_create(int ndims, const uint64_t* num_cells, const double* velocity
     return tfail(FDB_ERR_NODEV, "no HIP device available");
   fdb_tape* t = new fdb_tape;
   t->ndims = ndims;
-  for (int j = 0; j < ndims; ++j) {
-    t->n[j] = num_cells[j];
-    t->v[j] = velocity[j];
-    t->len[j] = lengths[j];
-  }
-  t->dt = delta_time;
   t->nsteps = nsteps;
   t->f_dtype = f_dtype;
   t->lam_dtype = lam_dtype;
   t->g = g;
   t->fbytes = (size_t)g.ntot * fdb::dtype_size(f_dtype);
   t->lbytes = (size_t)g.ntot * fdb::dtype_size(lam_dtype);
-  // Fixed: base + two working buffers + one adjoint buffer.
+  // Fixed: the initial state, two scratch buffers, one adjoint buffer.
   const uint64_t fixed = 3 * (uint64_t)t->fbytes + t->lbytes;
   if (max_bytes < fixed) {
     tape_free(t);
@@ -226,8 +302,8 @@ int fdb_tape_create(int ndims, const uint64_t* num_cells, const double* velocity
   };
   hipError_t e;
   if ((e = hipMalloc(&t->base, t->fbytes)) != hipSuccess) return fail_free(e, "hipMalloc base");
-  for (void*& w : t->work)
-    if ((e = hipMalloc(&w, t->fbytes)) != hipSuccess) return fail_free(e, "hipMalloc work");
+  for (void*& w : t->scratch)
+    if ((e = hipMalloc(&w, t->fbytes)) != hipSuccess) return fail_free(e, "hipMalloc scratch");
   if ((e = hipMalloc(&t->lam_tmp, t->lbytes)) != hipSuccess) return fail_free(e, "hipMalloc lam");
   t->slots.resize(t->snaps, nullptr);
   for (void*& s : t->slots)
@@ -244,33 +320,20 @@ int fdb_tape_forward(fdb_tape_t* t, const void* f0, void* fT, void* stream) {
   t->fwd_steps = 0;
   t->adj_steps = 0;
   TAPE_HIP(hipMemcpyAsync(t->base, f0, t->fbytes, hipMemcpyDeviceToDevice, s));
-  TAPE_HIP(hipMemcpyAsync(t->work[0], f0, t->fbytes, hipMemcpyDeviceToDevice, s));
-  t->wcur = 0;
+  t->w = t->base;
   t->wstep = 0;
   t->stack_steps.assign(1, 0);
-  // The first descent of the plan: advances and stores, up to the first
-  // restore or adjoint action.
-  size_t i = 0;
-  for (; i < t->plan.acts.size(); ++i) {
-    const auto& a = t->plan.acts[i];
-    if (a.first == FDB_CK_ADVANCE) {
-      if (int rc = advance(t, a.second, s)) return rc;
-    } else if (a.first == FDB_CK_STORE) {
-      TAPE_HIP(hipMemcpyAsync(t->slots[t->stack_steps.size() - 1], t->work[t->wcur], t->fbytes,
-                              hipMemcpyDeviceToDevice, s));
-      t->stack_steps.push_back(t->wstep);
-    } else {
-      break;
-    }
-  }
-  t->resume = i;
-  // Continue to state nsteps-1 in the working buffers (the backward's
-  // first adjoint needs it), then one step into fT.
-  if (int rc = advance(t, t->nsteps - 1 - t->wstep, s)) return rc;
-  void* const last[2] = {t->work[t->wcur], fT};
-  int lc = 0;
-  TAPE_HIP(fdb::launch_advect_pass(t->f_dtype, t->g, last, &lc, 1, s, nullptr));
-  t->fwd_steps += 1;
+  // The first descent of the plan (advances and stores), then on to state
+  // nsteps-1 (the backward's first adjoint needs it) and one step into fT.
+  size_t stop = 0;
+  if (int rc = run_plan(t, 0, true, s, nullptr, nullptr, nullptr, &stop)) return rc;
+  t->resume = stop;
+  if (int rc = advance(t, t->nsteps - 1 - t->wstep, nullptr, s)) return rc;
+  const void* keep = t->w;
+  const int keep_step = t->wstep;
+  if (int rc = step_into(t, fT, s)) return rc;
+  t->w = keep;  // fT is the caller's; the working state stays at nsteps-1
+  t->wstep = keep_step;
   TAPE_HIP(hipStreamSynchronize(s));
   t->forward_done = true;
   return FDB_OK;
@@ -283,13 +346,14 @@ int fdb_tape_backward(fdb_tape_t* t, const void* lam_T, void* lam0, double* grad
   if (lam_T == lam0) return tfail(FDB_ERR_INVALID, "tape_backward: lam_T and lam0 must differ");
   if (!t->forward_done) return tfail(FDB_ERR_INVALID, "tape_backward: run tape_forward first");
   hipStream_t s = (hipStream_t)stream;
-  // Partial-sum scratch for the adjoint kernel, sized for every buffer
-  // pairing the sweep uses.
+  // Partial-sum scratch of the adjoint kernel, sized for every buffer
+  // pairing the sweep uses (f_t is always a tape buffer).
   int64_t nb = 0;
-  nb = std::max(nb, fdb::adjoint_blocks(t->lam_dtype, t->g, lam_T, lam0));
-  nb = std::max(nb, fdb::adjoint_blocks(t->lam_dtype, t->g, lam_T, t->lam_tmp));
-  nb = std::max(nb, fdb::adjoint_blocks(t->lam_dtype, t->g, t->lam_tmp, lam0));
-  nb = std::max(nb, fdb::adjoint_blocks(t->lam_dtype, t->g, lam0, t->lam_tmp));
+  const void* pairs[4][2] = {{lam_T, lam0}, {lam_T, t->lam_tmp}, {t->lam_tmp, lam0},
+                             {lam0, t->lam_tmp}};
+  for (auto& p : pairs)
+    nb = std::max(nb, fdb::adjoint_blocks(t->lam_dtype, t->f_dtype, t->g, p[0],
+                                          const_cast<void*>(p[1]), t->base));
   const size_t pb = (size_t)nb * 6 * sizeof(double);
   if (pb > t->partials_bytes) {
     if (t->partials) TAPE_HIP(hipFree(t->partials));
@@ -300,53 +364,8 @@ int fdb_tape_backward(fdb_tape_t* t, const void* lam_T, void* lam0, double* grad
   }
   TAPE_HIP(fdb::launch_zero(t->acc, 6, s));
   const void* lam_next = lam_T;
-  int done = 0;  // adjoint steps taken
-  const auto& acts = t->plan.acts;
-  for (size_t i = t->resume; i < acts.size(); ++i) {
-    const auto& a = acts[i];
-    switch (a.first) {
-      case FDB_CK_ADVANCE:
-        if (int rc = advance(t, a.second, s)) return rc;
-        break;
-      case FDB_CK_STORE:
-        TAPE_HIP(hipMemcpyAsync(t->slots[t->stack_steps.size() - 1], t->work[t->wcur],
-                                t->fbytes, hipMemcpyDeviceToDevice, s));
-        t->stack_steps.push_back(t->wstep);
-        break;
-      case FDB_CK_POP:
-        t->stack_steps.pop_back();
-        break;
-      case FDB_CK_RESTORE: {
-        // Skip the copy (and the advance after it) when the working
-        // buffer already holds the state they would produce.
-        const int top = t->stack_steps.back();
-        const bool adv = i + 1 < acts.size() && acts[i + 1].first == FDB_CK_ADVANCE;
-        const int target = top + (adv ? acts[i + 1].second : 0);
-        if (t->wstep == target) {
-          if (adv) ++i;
-          break;
-        }
-        TAPE_HIP(hipMemcpyAsync(t->work[t->wcur], top_buffer(t), t->fbytes,
-                                hipMemcpyDeviceToDevice, s));
-        t->wstep = top;
-        break;
-      }
-      case FDB_CK_ADJOINT: {
-        if (t->wstep != a.second)
-          return tfail(FDB_ERR_INVALID, "tape_backward: schedule out of step (internal)");
-        ++done;
-        // The last adjoint step writes into lam0.
-        void* dst = ((t->nsteps - done) % 2 == 0) ? lam0 : t->lam_tmp;
-        TAPE_HIP(fdb::launch_adjoint_step(t->lam_dtype, t->f_dtype, t->g, lam_next, dst,
-                                          t->work[t->wcur], t->partials, t->acc, s));
-        lam_next = dst;
-        ++t->adj_steps;
-        break;
-      }
-      default:
-        return tfail(FDB_ERR_INVALID, "tape_backward: bad action (internal)");
-    }
-  }
+  int done = 0;
+  if (int rc = run_plan(t, t->resume, false, s, &lam_next, lam0, &done, nullptr)) return rc;
   if (done != t->nsteps) return tfail(FDB_ERR_INVALID, "tape_backward: incomplete (internal)");
   double acc[6];
   TAPE_HIP(hipMemcpyAsync(acc, t->acc, sizeof(acc), hipMemcpyDeviceToHost, s));

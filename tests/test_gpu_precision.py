@@ -34,6 +34,36 @@ def test_reduced_precision_bit_exact_vs_oracle(manifest, oracle, dtype):
         up.close()
 
 
+@pytest.mark.parametrize("dtype", ["f32", "bf16"])
+def test_special_values_match_oracle(oracle, dtype):
+    """Subnormals, values near the bf16/f32 overflow threshold, rounding
+    ties, +-Inf and NaN: the device rounding (v_cvt_pk_bf16_f32 for bf16)
+    must agree with the oracle's software round-to-nearest-even.  NaN
+    payloads differ between CPU and GPU arithmetic, so NaN positions are
+    compared, and every other value bit for bit."""
+    from fidibench_amd import Upwind
+    shape, v = (6, 8, 64), (0.35, -0.25, 0.3)
+    rng = np.random.default_rng(11)
+    f = rng.standard_normal(shape)
+    flat = f.reshape(-1)
+    specials = [1e-39, -3e-40, 1e-45, 3.38e38, -3.39e38, 3.3895e38, 1.00390625, 1.01171875,
+                1.0 + 2 ** -8, 1.0 + 3 * 2 ** -9, 65504.0, 2.0 ** -126, -(2.0 ** -133)]
+    flat[5:5 + len(specials)] = specials
+    flat[300] = np.inf
+    flat[1000] = -np.inf
+    flat[2000] = np.nan
+    up = Upwind(v, (1, 1, 1), shape, dtype=dtype)
+    up.set_field(f)
+    start = up.get_field()
+    ref = oracle.advect(start, v, (1, 1, 1), 0.01, 3)
+    up.advect(3, 0.01)
+    got = up.get_field()
+    wide = (lambda a: oracle.bf16_to_f32(a)) if dtype == "bf16" else (lambda a: a)
+    nan_g, nan_r = np.isnan(wide(got)), np.isnan(wide(ref))
+    assert np.array_equal(nan_g, nan_r)
+    assert np.array_equal(bits(got)[~nan_g], bits(ref)[~nan_r])
+
+
 def _errors(x, ref):
     d = x.astype(np.float64) - ref
     return {"max_abs": float(np.abs(d).max()), "l2": float(np.sqrt(np.mean(d * d))),
