@@ -78,6 +78,22 @@ __device__ inline void store_pack(TS* p, const typename St<TS>::M (&in)[VEC]) {
   *reinterpret_cast<Pack<TS, VEC>*>(p) = pk;
 }
 
+// Value of the neighbouring lane through a DPP wavefront shift (a VALU
+// move, no LDS round trip): lane_below gives lane L the value of lane
+// L-1, lane_above that of lane L+1; lane 0 (resp. 63) keeps its own.
+template <int CTRL, typename T>
+__device__ inline T dpp_shift(T x) {
+  static_assert(sizeof(T) % 4 == 0, "32-bit lanes");
+  constexpr int W = sizeof(T) / 4;
+  struct Words { int w[W]; };
+  Words a = __builtin_bit_cast(Words, x), r;
+#pragma unroll
+  for (int i = 0; i < W; ++i) r.w[i] = __builtin_amdgcn_update_dpp(a.w[i], a.w[i], CTRL, 0xf, 0xf, false);
+  return __builtin_bit_cast(T, r);
+}
+template <typename T> __device__ inline T lane_below(T x) { return dpp_shift<0x138>(x); }  // wave_shr:1
+template <typename T> __device__ inline T lane_above(T x) { return dpp_shift<0x130>(x); }  // wave_shl:1
+
 __device__ inline int64_t wrap(int64_t x, int64_t n) {
   return x < 0 ? x + n : (x >= n ? x - n : x);
 }
@@ -482,17 +498,17 @@ __device__ inline void rows_tile(const TS* __restrict__ in, TS* __restrict__ out
       load_f(p + step, fn);
     }
     double ps[3] = {0.0, 0.0, 0.0};
-    // axis-2 neighbours: shuffles are done by all lanes together
+    // axis-2 neighbours: lane shifts are done by all lanes together
     M n2[R][VEC];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       if (lo2) {
-        const M sh = __shfl_up(cur[r][VEC - 1], 1);
+        const M sh = lane_below(cur[r][VEC - 1]);
         n2[r][0] = edge ? ed[r] : sh;
 #pragma unroll
         for (int v = 1; v < VEC; ++v) n2[r][v] = cur[r][v - 1];
       } else {
-        const M sh = __shfl_down(cur[r][0], 1);
+        const M sh = lane_above(cur[r][0]);
         n2[r][VEC - 1] = edge ? ed[r] : sh;
 #pragma unroll
         for (int v = 0; v < VEC - 1; ++v) n2[r][v] = cur[r][v + 1];
@@ -606,6 +622,137 @@ __global__ __launch_bounds__(64 * WY) void rows_kernel(const TS* __restrict__ in
   if constexpr (ADJ) {
     if (ft != nullptr) block_reduce_vacc<64 * WY>(acc, partials);
   }
+}
+
+// ------------------------------------------ temporally blocked forward --
+//
+// K steps per pass over HBM.  Each wave owns an extended tile: 64 lanes
+// along axis 2 (one cell each) x R rows of axis 1 (registers), reaching
+// K cells past its output region on the upwind side of both axes; it
+// outputs (64-K) x (R-K) cells per plane.  The wave marches along axis 0
+// from upwind to downwind and, at every plane, computes levels 1..K of
+// the step in registers; level l of plane p needs level l-1 of plane p
+// (in-register rows, neighbouring lanes) and of plane p-upwind (kept per
+// level in prev[l-1]).  It starts K planes before its chunk so that every
+// level is valid when output begins.  Cells within l of the upwind tile
+// edges hold wrong values at level l, but they never reach an output cell
+// (an output cell's level-K value depends only on cells at most K away
+// upwind).  Each value is produced by the same operations on the same
+// inputs as K separate steps, so the result is bit-identical.
+struct TbArgs {
+  int64_t n0, n1, n2;
+  int u0, u1, u2;
+  int64_t chunk;               // output planes per tile along axis 0
+  int64_t tiles2, tiles1, ntiles;
+};
+
+template <typename TS, int R, int K>
+__device__ inline void tb_tile(const TS* __restrict__ in, TS* __restrict__ out,
+                               const TbArgs& a, const Coef<typename St<TS>::M>& cf, int64_t t) {
+  using M = typename St<TS>::M;
+  const int lane = threadIdx.x;
+  const int64_t t2 = t % a.tiles2;
+  const int64_t t1 = (t / a.tiles2) % a.tiles1;
+  const int64_t ch = t / (a.tiles2 * a.tiles1);
+
+  // Columns: output columns o2 .. o2+63-K; the halo is on the upwind side.
+  const int64_t o2 = t2 * (64 - K);
+  const bool lo2 = a.u2 < 0;
+  const int64_t cl = lo2 ? o2 - K + lane : o2 + lane;  // unwrapped column of this lane
+  const int64_t col = wrap(cl, a.n2);
+  const bool col_out = (lo2 ? lane >= K : lane < 64 - K) && cl >= 0 && cl < a.n2;
+  // Rows: output rows o1 .. o1+R-1-K.
+  const int64_t o1 = t1 * (R - K);
+  const bool lo1 = a.u1 < 0;
+  int32_t roff[R];  // in-plane offsets (the dispatch requires n1*n2 < 2^31)
+  bool row_out[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int64_t rl = lo1 ? o1 - K + r : o1 + r;
+    roff[r] = (int32_t)(wrap(rl, a.n1) * a.n2 + col);
+    row_out[r] = (lo1 ? r >= K : r < R - K) && rl >= 0 && rl < a.n1;
+  }
+  // Planes: output p_lo .. p_hi-1; march K planes earlier (upwind side).
+  const int64_t plane = a.n1 * a.n2;
+  const int64_t p_lo = ch * a.chunk;
+  const int64_t p_hi = (p_lo + a.chunk < a.n0) ? p_lo + a.chunk : a.n0;
+  const int64_t step = a.u0 < 0 ? 1 : -1;
+  const int64_t first = a.u0 < 0 ? p_lo - K : p_hi - 1 + K;
+  const int64_t cnt = p_hi - p_lo + K;
+
+  M prev[K][R];
+#pragma unroll
+  for (int l = 0; l < K; ++l)
+#pragma unroll
+    for (int r = 0; r < R; ++r) prev[l][r] = M(0);
+  M x[R], nx[R];
+  {
+    const TS* base = in + wrap(first, a.n0) * plane;
+#pragma unroll
+    for (int r = 0; r < R; ++r) x[r] = St<TS>::ld(base[roff[r]]);
+  }
+  int64_t p = first;
+  for (int64_t m = 0; m < cnt; ++m) {
+    if (m + 1 < cnt) {
+      const TS* base = in + wrap(p + step, a.n0) * plane;
+#pragma unroll
+      for (int r = 0; r < R; ++r) nx[r] = St<TS>::ld(base[roff[r]]);
+    }
+#pragma unroll
+    for (int l = 0; l < K; ++l) {
+      // Level l+1 from level l, in place.  Rows run against the axis-1
+      // upwind direction so that row r still sees the old row r+u1.
+      if (lo1) {
+#pragma unroll
+        for (int r = R - 1; r >= 0; --r) {
+          const M xv = x[r];
+          const M n1 = r > 0 ? x[r > 0 ? r - 1 : 0] : xv;
+          const M n2 = lo2 ? lane_below(xv) : lane_above(xv);
+          M g = xv;
+          g = g - cf.c[0] * (prev[l][r] - xv);
+          g = g - cf.c[1] * (n1 - xv);
+          g = g - cf.c[2] * (n2 - xv);
+          prev[l][r] = xv;
+          x[r] = St<TS>::ld(St<TS>::st(g));  // each level as stored (bf16 rounds)
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const M xv = x[r];
+          const M n1 = r < R - 1 ? x[r < R - 1 ? r + 1 : R - 1] : xv;
+          const M n2 = lo2 ? lane_below(xv) : lane_above(xv);
+          M g = xv;
+          g = g - cf.c[0] * (prev[l][r] - xv);
+          g = g - cf.c[1] * (n1 - xv);
+          g = g - cf.c[2] * (n2 - xv);
+          prev[l][r] = xv;
+          x[r] = St<TS>::ld(St<TS>::st(g));  // each level as stored (bf16 rounds)
+        }
+      }
+    }
+    if (m >= K && col_out) {
+      TS* ob = out + wrap(p, a.n0) * plane;
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (row_out[r]) __builtin_nontemporal_store(St<TS>::st(x[r]), ob + roff[r]);
+    }
+    if (m + 1 < cnt) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) x[r] = nx[r];
+    }
+    p += step;
+  }
+}
+
+template <typename TS, int R, int K, int WY>
+__global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
+                                                     TS* __restrict__ out, TbArgs a,
+                                                     Coef<typename St<TS>::M> cf) {
+  int64_t b = blockIdx.x;
+  const int64_t nb = gridDim.x;
+  if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);  // XCD-contiguous tiles
+  const int64_t t = b * WY + threadIdx.y;                    // one tile per wave
+  if (t < a.ntiles) tb_tile<TS, R, K>(in, out, a, cf, t);
 }
 
 // ------------------------------------------------------------ dispatch ---
@@ -735,6 +882,117 @@ hipError_t rows_dispatch(const Grid3& g, const TS* in, TS* out, const int dirs[3
   return hipGetLastError();
 }
 
+// Temporally blocked pass: K steps in one launch.  Returns
+// hipErrorNotSupported when the shape does not fit (the caller then runs
+// one-step launches): all three axes present, n2 >= 64, n1 >= R, n0 > K.
+// FDB_TB_CFG_<bytes>_<K>="R,WY,WAVES" overrides the tile rows, waves per
+// block and the target number of tiles (waves) per launch.
+// WAVES = 0: choose the chunk count from the kernel's occupancy (below).
+struct TbCfg {
+  int R, WY, waves;
+};
+
+TbCfg tb_cfg(int bytes, int K) {
+  // 512^3 sweeps: R=24 rows for FP64 at K=4 (one wave per SIMD), 16 rows
+  // otherwise; two waves per block.
+  TbCfg c{16, 2, 0};
+  if (bytes == 8 && K >= 4) c.R = 24;
+  char name[48];
+  snprintf(name, sizeof(name), "FDB_TB_CFG_%d_%d", bytes, K);
+  if (const char* s = getenv(name)) {
+    int r, w, n;
+    if (sscanf(s, "%d,%d,%d", &r, &w, &n) == 3) c = TbCfg{r, w, n};
+  }
+  return c;
+}
+
+// Number of axis-0 chunks for `tiles` column tiles: every wave is one
+// tile, the device holds `capacity` waves at once, and a chunk of c
+// planes costs c + K plane visits.  Pick the count that wastes least on
+// the last, partly filled round of waves and on warm-up planes.
+int64_t tb_chunks(int64_t tiles, int64_t n0, int K, int64_t capacity) {
+  int64_t best = 1;
+  double best_eff = -1.0;
+  for (int64_t nch = 1; nch <= 64 && nch <= n0; ++nch) {
+    const int64_t chunk = (n0 + nch - 1) / nch;
+    const int64_t real = (n0 + chunk - 1) / chunk;
+    const int64_t waves = tiles * real;
+    const int64_t rounds = (waves + capacity - 1) / capacity;
+    const double eff = (double)waves / (double)(rounds * capacity) *
+                       (double)n0 / (double)(real * (chunk + K));
+    if (eff > best_eff + 1e-9) {
+      best_eff = eff;
+      best = real;
+    }
+  }
+  return best;
+}
+
+int device_cus() {
+  static int cus = [] {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      n = 256;
+    return n > 0 ? n : 256;
+  }();
+  return cus;
+}
+
+template <typename TS>
+hipError_t tb_dispatch(const Grid3& g, const TS* in, TS* out, int K, hipStream_t s) {
+  using M = typename St<TS>::M;
+  // 2-byte storage: the one-step kernel moves 16 bytes per lane and is
+  // faster than any fused pass with one cell per lane (measured).
+  if (sizeof(TS) < 4) return hipErrorNotSupported;
+  const TbCfg cfg = tb_cfg((int)sizeof(TS), K);
+  const int R = cfg.R, WY = cfg.WY;
+  if (!(g.act[0] && g.act[1] && g.act[2]) || g.n[2] < 64 || g.n[1] < R || g.n[0] <= K ||
+      R <= K || g.n[1] * g.n[2] >= (int64_t)1 << 31)
+    return hipErrorNotSupported;
+  TbArgs a;
+  a.n0 = g.n[0];
+  a.n1 = g.n[1];
+  a.n2 = g.n[2];
+  a.u0 = g.up[0];
+  a.u1 = g.up[1];
+  a.u2 = g.up[2];
+  a.tiles2 = (a.n2 + (64 - K) - 1) / (64 - K);
+  a.tiles1 = (a.n1 + (R - K) - 1) / (R - K);
+  const int64_t tiles = a.tiles2 * a.tiles1;
+  Coef<M> cf;
+  for (int j = 0; j < 3; ++j) cf.c[j] = (M)g.c[j];
+  bool launched = false;
+#define FDB_TB(RR, KK, W)                                                                   \
+  if (!launched && R == RR && K == KK && WY == W) {                                         \
+    auto kern = tb_kernel<TS, RR, KK, W>;                                                   \
+    int64_t nch;                                                                            \
+    if (cfg.waves > 0) {                                                                    \
+      nch = std::max<int64_t>(1, (cfg.waves + tiles / 2) / tiles);                          \
+    } else {                                                                                \
+      int per_cu = 0;                                                                       \
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * W, 0) !=         \
+              hipSuccess || per_cu < 1)                                                     \
+        per_cu = 1;                                                                         \
+      nch = tb_chunks(tiles, a.n0, KK, (int64_t)per_cu * W * device_cus());                \
+    }                                                                                       \
+    a.chunk = (a.n0 + nch - 1) / nch;                                                       \
+    a.ntiles = tiles * ((a.n0 + a.chunk - 1) / a.chunk);                                    \
+    const int64_t nb = (a.ntiles + W - 1) / W;                                              \
+    if (nb > 0x7fffffffLL) return hipErrorInvalidConfiguration;                             \
+    hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(64, W), 0, s, in, out, a, cf);        \
+    launched = true;                                                                        \
+  }
+  FDB_TB(16, 2, 2)
+  FDB_TB(16, 3, 2)
+  FDB_TB(16, 4, 2)
+  FDB_TB(24, 4, 2)
+  FDB_TB(24, 4, 1)
+#undef FDB_TB
+  if (!launched) return hipErrorNotSupported;
+  return hipGetLastError();
+}
+
 template <typename TS>
 hipError_t step_typed(const Grid3& g, const void* in, void* out, hipStream_t s) {
   return rows_dispatch<TS, TS, false>(g, (const TS*)in, (TS*)out, g.up, (const TS*)nullptr,
@@ -805,20 +1063,44 @@ hipError_t launch_advect_pass(int dtype, const Grid3& g, void* const bufs[2], in
                               hipStream_t s, int* launches) {
   if (bufs[0] == bufs[1] || (*cur != 0 && *cur != 1)) return hipErrorInvalidValue;
   hipError_t err = hipSuccess;
-  for (int i = 0; i < k; ++i) {
+  int left = k, n = 0;
+  while (left > 0) {
     const void* src = bufs[*cur];
     void* dst = bufs[1 - *cur];
-    switch (dtype) {
-      case FDB_F64: err = step_typed<double>(g, src, dst, s); break;
-      case FDB_F32: err = step_typed<float>(g, src, dst, s); break;
-      case FDB_BF16: err = step_typed<uint16_t>(g, src, dst, s); break;
-      default: return hipErrorInvalidValue;
+    int done = 0;
+    // Temporally blocked pass with the deepest supported K <= left.
+    for (int K : {4, 3, 2}) {
+      if (K > left) continue;
+      switch (dtype) {
+        case FDB_F64: err = tb_dispatch<double>(g, (const double*)src, (double*)dst, K, s); break;
+        case FDB_F32: err = tb_dispatch<float>(g, (const float*)src, (float*)dst, K, s); break;
+        case FDB_BF16:
+          err = tb_dispatch<uint16_t>(g, (const uint16_t*)src, (uint16_t*)dst, K, s);
+          break;
+        default: return hipErrorInvalidValue;
+      }
+      if (err == hipSuccess) {
+        done = K;
+        break;
+      }
+      if (err != hipErrorNotSupported) return err;
     }
-    if (err != hipSuccess) return err;
+    if (done == 0) {
+      switch (dtype) {
+        case FDB_F64: err = step_typed<double>(g, src, dst, s); break;
+        case FDB_F32: err = step_typed<float>(g, src, dst, s); break;
+        case FDB_BF16: err = step_typed<uint16_t>(g, src, dst, s); break;
+        default: return hipErrorInvalidValue;
+      }
+      if (err != hipSuccess) return err;
+      done = 1;
+    }
     *cur = 1 - *cur;
+    left -= done;
+    ++n;
   }
-  if (launches) *launches = k;
-  return err;
+  if (launches) *launches = n;
+  return hipSuccess;
 }
 
 hipError_t launch_adjoint_step(int lam_dtype, int f_dtype, const Grid3& g, const void* lam_next,
