@@ -37,8 +37,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "cell-updates/s & %HBM roofline at 512³: FP64 plain vs FP32/BF16 vs fwd+adjoint"
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
-BYTES = {"f64": 8, "f32": 4, "bf16": 2}
-ALL = ("f64", "f32", "bf16", "mixed", "fwdadj")
+BYTES = {"f64": 8, "f32": 4, "bf16": 2, "bf16sr": 2}
+ALL = ("f64", "f32", "bf16", "mixed", "fwdadj", "adjlarge")
 
 
 def parse():
@@ -134,24 +134,28 @@ def compare(a_up, b_up):
 
 
 def mixed_precision(n=1024, steps=200):
-    """The Gaussian-bubble workload in FP64, FP32 and BF16 storage."""
+    """The Gaussian-bubble workload in FP64, FP32 and BF16 storage (round to
+    nearest even, and stochastic rounding); FP64/FP32 fuse 4 steps per pass
+    (same results)."""
     from fidibench_amd import configs as K
     w = K.gauss1024(n=n, steps=steps)
     res, ups = {}, {}
-    for dt_name in ("f64", "f32", "bf16"):
+    for dt_name in ("f64", "f32", "bf16", "bf16sr"):
         from fidibench_amd import Upwind
         up = Upwind(w.velocity, w.lengths, w.num_cells, dtype=dt_name)
+        up.set_time_block(4)
         up.init_field(w.init_kind, w.init_params, seed=w.init_seed)
         m0 = up.mass()
         sync()
         ms, launches = up.advect_timed(steps, w.dt)
         m1 = up.mass()
         res[dt_name] = {"cell_updates_per_s": w.ntot * steps / (ms * 1e-3), "ms": ms,
+                        "launches": launches,
                         "hbm_frac_compulsory": w.ntot * steps / (ms * 1e-3) * 2 *
                         BYTES[dt_name] / HBM_PEAK,
                         "mass_drift_rel": abs(m1 - m0) / abs(m0)}
         ups[dt_name] = up
-    for dt_name in ("f32", "bf16"):
+    for dt_name in ("f32", "bf16", "bf16sr"):
         res[dt_name]["error_vs_f64"] = compare(ups[dt_name], ups["f64"])
     for up in ups.values():
         up.close()
@@ -194,6 +198,43 @@ def forward_adjoint(w, steps, f_dtype, fwd_ms):
             "cell_steps_per_s": w.ntot * steps / (tot * 1e-3), "snapshots": st["snapshots"],
             "forward_steps": st["forward_steps"], "adjoint_steps": st["adjoint_steps"],
             "grad_v": gv}
+
+
+def large_adjoint(shape=(1024, 1024, 768), steps=400):
+    """The checkpointed gradient at 1024x1024x768 x 400 steps: BF16-storage
+    forward with FP32 adjoint, against the same gradient in FP64; time
+    ratio to the plain BF16 forward over the same steps."""
+    import torch
+    from fidibench_amd import _lib as L, configs as K
+    from fidibench_amd.workloads import advect, compare, init_field, misfit_gradient
+    w = K.misfit_large(shape, steps)
+    wt = K.Workload("misfit_large_target", w.num_cells, steps, (0.8, 0.3, -0.5), w.init_kind,
+                    w.init_params, dt_velocity=w.dt_velocity)
+    target = advect(init_field(wt, L.F64), wt.velocity, wt, steps)
+    free = torch.cuda.mem_get_info()[0]
+    g64 = misfit_gradient(w, target, L.F64, L.F64, int(free * 0.7))
+    lam64, gv64 = g64.pop("grad_f0"), g64.pop("grad_v")
+    free = torch.cuda.mem_get_info()[0]
+    g16 = misfit_gradient(w, target, L.BF16_SR, L.F32, int(free * 0.8))
+    lam16, gv16 = g16.pop("grad_f0"), g16.pop("grad_v")
+    err = compare(lam16, lam64)
+    del lam16, lam64
+    f0 = init_field(w, L.BF16)
+    sync()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    advect(f0, w.velocity, w, steps, 1)
+    e1.record()
+    sync()
+    fwd_s = e0.elapsed_time(e1) * 1e-3
+    return {"workload": (f"misfit_large: {shape[0]}x{shape[1]}x{shape[2]}, {steps} steps, "
+                         "8 Gaussian bubbles a*exp(-|x-c|^2/w^2), v=(0.9,0.25,-0.45), target "
+                         "at v*=(0.8,0.3,-0.5)"),
+            "bf16_fp32": {**g16, "ratio_to_plain_bf16_forward": g16["seconds"] / fwd_s},
+            "fp64": g64, "plain_bf16_forward_s": fwd_s,
+            "grad_f0_error_vs_fp64": err,
+            "grad_v_rel_error_vs_fp64": [abs(a - b) / abs(b) for a, b in zip(gv16, gv64)],
+            "grad_v_fp64": gv64}
 
 
 def pmc_traffic(kernel_key):
@@ -255,6 +296,8 @@ def main():
         res["fwd_adj_f64"] = forward_adjoint(w, a.steps, "f64", fwd_ms)
         f32_ms = res.get("f32_k1", {}).get("ms")
         res["fwd_adj_f32"] = forward_adjoint(w, a.steps, "f32", f32_ms)
+    if "adjlarge" in only:
+        res["adjoint_large"] = large_adjoint()
 
     parity = None
     try:

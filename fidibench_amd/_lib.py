@@ -11,10 +11,10 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libfdb_upwind.so")
 
-F64, F32, BF16 = 0, 1, 2
-DTYPES = {"f64": F64, "f32": F32, "bf16": BF16}
+F64, F32, BF16, BF16_SR = 0, 1, 2, 3
+DTYPES = {"f64": F64, "f32": F32, "bf16": BF16, "bf16sr": BF16_SR}
 DTYPE_NAMES = {v: k for k, v in DTYPES.items()}
-ITEMSIZE = {F64: 8, F32: 4, BF16: 2}
+ITEMSIZE = {F64: 8, F32: 4, BF16: 2, BF16_SR: 2}
 
 INIT_DELTA, INIT_SPHERE, INIT_GAUSS, INIT_GAUSS_SUM, INIT_UNIFORM = range(5)
 CK_ADVANCE, CK_STORE, CK_RESTORE, CK_POP, CK_ADJOINT = 1, 2, 3, 4, 5
@@ -46,6 +46,7 @@ SIGNATURES = {
     "fdb_upwind_init_field": (_i, [_vp, _i, _dp, _i, C.c_uint64]),
     "fdb_upwind_mass": (_i, [_vp, _dp]),
     "fdb_upwind_set_time_block": (_i, [_vp, _i]),
+    "fdb_upwind_set_seed": (_i, [_vp, C.c_uint64]),
     "fdb_upwind_advect_timed": (_i, [_vp, _i, C.c_double, C.POINTER(C.c_float), C.POINTER(_i)]),
     "fdb_advect_dev": (_i, [_i, _u64p, _dp, _dp, C.c_double, _i, _vp, _vp, _i, _i, _vp,
                             C.POINTER(_i)]),

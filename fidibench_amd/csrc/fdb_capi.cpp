@@ -50,7 +50,9 @@ int hip_fail(hipError_t e, const char* what) {
     if (_e != hipSuccess) return hip_fail(_e, #call);  \
   } while (0)
 
-bool valid_dtype(int d) { return d == FDB_F64 || d == FDB_F32 || d == FDB_BF16; }
+bool valid_dtype(int d) {
+  return d == FDB_F64 || d == FDB_F32 || d == FDB_BF16 || d == FDB_BF16_SR;
+}
 
 // Host copies of the device conversions (bf16: round to nearest even on
 // the binary32 value; a double is first rounded to float).
@@ -85,6 +87,8 @@ inline void h_narrow(int dtype, void* p, uint64_t i, double x) {
 // Same-type copies must not round-trip through double (bf16 NaN payloads,
 // signed zeros are kept either way, but a plain copy is exact and faster).
 void h_convert(int sd, const void* src, int dd, void* dst, uint64_t n) {
+  sd = fdb::storage_dtype(sd);
+  dd = fdb::storage_dtype(dd);
   if (sd == dd) {
     std::memcpy(dst, src, n * fdb::dtype_size(sd));
     return;
@@ -150,6 +154,8 @@ struct fdb_upwind {
   hipStream_t stream = nullptr;
   double* partials = nullptr;  // 6 * kReduceBlocks + 6
   int time_block = 1;
+  uint64_t seed = 0;        // stochastic rounding (FDB_BF16_SR)
+  uint64_t steps_done = 0;  // steps advanced since creation or set_seed
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   void* stage = nullptr;  // pinned host staging
 };
@@ -197,7 +203,9 @@ int run_advect(fdb_upwind* h, int nsteps, double dt, int* launches) {
   while (left > 0) {
     const int k = std::min(left, h->time_block);
     int l = 0;
-    FDB_HIP(fdb::launch_advect_pass(h->dtype, g, h->buf, &h->cur, k, h->stream, &l));
+    FDB_HIP(fdb::launch_advect_pass(h->dtype, g, h->buf, &h->cur, k, h->stream, &l,
+                                    h->steps_done, h->seed));
+    h->steps_done += (uint64_t)k;
     total += l;
     left -= k;
   }
@@ -400,6 +408,13 @@ int fdb_upwind_mass(fdb_upwind_t* h, double* out) {
   FDB_HIP(hipMemcpyAsync(r, out2, sizeof(r), hipMemcpyDeviceToHost, h->stream));
   FDB_HIP(hipStreamSynchronize(h->stream));
   *out = r[0] + r[1];
+  return FDB_OK;
+}
+
+int fdb_upwind_set_seed(fdb_upwind_t* h, uint64_t seed) {
+  if (!h) return fail(FDB_ERR_INVALID, "set_seed: null handle");
+  h->seed = seed;
+  h->steps_done = 0;
   return FDB_OK;
 }
 

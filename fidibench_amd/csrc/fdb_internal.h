@@ -49,8 +49,14 @@ constexpr int kMaxTimeBlock = 8;
 // launch of the streaming kernel; k > 1 runs the temporally blocked
 // kernel when the shape fits it, else k streaming launches.  *launches
 // (optional) receives the number of kernel launches.
+// For FDB_BF16_SR the stochastic rounding of step number step0 + i uses
+// a key hashed from (seed, step0 + i).
 hipError_t launch_advect_pass(int dtype, const Grid3& g, void* const bufs[2], int* cur, int k,
-                              hipStream_t s, int* launches);
+                              hipStream_t s, int* launches, uint64_t step0 = 0,
+                              uint64_t seed = 0);
+
+// The storage layout of a dtype (FDB_BF16_SR is stored as FDB_BF16).
+inline int storage_dtype(int dtype) { return dtype == 3 ? 2 : dtype; }
 
 // lam = A^T lam_next; if f_t != nullptr also adds the three velocity
 // gradient dot products into acc (device, 3 x {sum, comp}).

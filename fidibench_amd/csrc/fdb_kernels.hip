@@ -48,6 +48,29 @@ template <> struct St<uint16_t> {
   __device__ static uint16_t st(M x) { return __builtin_bit_cast(uint16_t, (__bf16)x); }
 };
 
+// Stochastic rounding of step results for bf16 storage (FDB_BF16_SR): 16
+// random bits are added below the kept bits of the binary32 value, then
+// truncated, so x rounds up with probability equal to the discarded
+// fraction (unbiased).  The bits are lowbias32 (C. Wellons' 32-bit integer
+// hash) of the cell's flat index mixed with a per-step key made on the
+// host from the seed and the step's index; Inf stays Inf, NaN stays quiet.
+__host__ __device__ inline uint32_t lowbias32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+__device__ inline uint16_t bf16_sr(float x, uint64_t cell, uint32_t key) {
+  const uint32_t b = __builtin_bit_cast(uint32_t, x);
+  if ((b & 0x7f800000u) == 0x7f800000u)
+    return (uint16_t)((b >> 16) | ((b & 0x007fffffu) ? 0x0040u : 0u));
+  const uint32_t r = lowbias32((uint32_t)(cell ^ (cell >> 32)) ^ key) & 0xffffu;
+  return (uint16_t)((b + r) >> 16);
+}
+
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
@@ -109,6 +132,7 @@ struct MarchArgs {
   int act0, act1, act2;
   int64_t chunk;            // planes per block along axis 0
   int64_t tiles2, tiles1;   // tiles along axis 2 and axis 1
+  uint32_t sr_key;          // stochastic-rounding key of this step (BF16_SR)
 };
 
 // Adds npart per-block {sum, comp} x3 partials into acc (device, 6
@@ -398,6 +422,21 @@ __device__ inline void store_pack_nt(TS* p, const typename St<TS>::M (&in)[VEC])
     *reinterpret_cast<R*>(p) = raw;
 }
 
+// bf16 store with stochastic rounding (cell = flat index of the first).
+template <int VEC, bool NT>
+__device__ inline void store_pack_sr(uint16_t* p, const float (&in)[VEC], uint64_t cell,
+                                     uint32_t key) {
+  Pack<uint16_t, VEC> pk;
+#pragma unroll
+  for (int v = 0; v < VEC; ++v) pk.v[v] = bf16_sr(in[v], cell + v, key);
+  using R = typename RawT<sizeof(Pack<uint16_t, VEC>)>::T;
+  const R raw = __builtin_bit_cast(R, pk);
+  if constexpr (NT)
+    __builtin_nontemporal_store(raw, reinterpret_cast<R*>(p));
+  else
+    *reinterpret_cast<R*>(p) = raw;
+}
+
 // Per-thread compensated accumulators of the three velocity-gradient dot
 // products (adjoint kernel only).
 struct VAcc {
@@ -413,7 +452,7 @@ struct VAcc {
 // the products f_t[i] * (lam_next[i + dir_j e_j] - lam_next[i]) summed per
 // plane in FP64 and added to acc (sum_i lam_next[i] (f_t[up_j(i)] - f_t[i])
 // by the periodic shift i -> i + dir_j e_j).
-template <typename TS, typename TF, int VEC, int R, int WY, bool NT, bool ADJ>
+template <typename TS, typename TF, int VEC, int R, int WY, bool NT, bool ADJ, bool SR>
 __device__ inline void rows_tile(const TS* __restrict__ in, TS* __restrict__ out,
                                  const MarchArgs& a, const Coef<typename St<TS>::M>& cf,
                                  int64_t b, const TF* __restrict__ ft, VAcc& acc) {
@@ -541,7 +580,13 @@ __device__ inline void rows_tile(const TS* __restrict__ in, TS* __restrict__ out
         }
         prev[r][v] = x;
       }
-      if (col_ok && r < nvalid) store_pack_nt<TS, VEC, NT>(obase + roff[r], o);
+      if (col_ok && r < nvalid) {
+        if constexpr (SR)
+          store_pack_sr<VEC, NT>(obase + roff[r], o, (uint64_t)(p * plane + roff[r] + i2c),
+                                 a.sr_key);
+        else
+          store_pack_nt<TS, VEC, NT>(obase + roff[r], o);
+      }
     }
     if constexpr (ADJ) {
       if (with_f) {
@@ -600,7 +645,7 @@ __device__ inline void block_reduce_vacc(const VAcc& acc, double* __restrict__ p
 // gets a contiguous range of tile numbers so tiles that share halo rows
 // and chunk-boundary planes share that XCD's L2.  With gridDim.x < ntiles
 // the grid is persistent: block (xcd, l) takes tiles start+l, start+l+n8..
-template <typename TS, typename TF, int VEC, int R, int WY, bool NT, bool ADJ>
+template <typename TS, typename TF, int VEC, int R, int WY, bool NT, bool ADJ, bool SR>
 __global__ __launch_bounds__(64 * WY) void rows_kernel(const TS* __restrict__ in,
                                                        TS* __restrict__ out, MarchArgs a,
                                                        Coef<typename St<TS>::M> cf,
@@ -614,10 +659,10 @@ __global__ __launch_bounds__(64 * WY) void rows_kernel(const TS* __restrict__ in
     const int64_t lo = xcd * per;
     const int64_t hi = lo + per < ntiles ? lo + per : ntiles;
     for (int64_t t = lo + l; t < hi; t += n8)
-      rows_tile<TS, TF, VEC, R, WY, NT, ADJ>(in, out, a, cf, t, ft, acc);
+      rows_tile<TS, TF, VEC, R, WY, NT, ADJ, SR>(in, out, a, cf, t, ft, acc);
   } else {
     for (int64_t t = blockIdx.x; t < ntiles; t += nb)
-      rows_tile<TS, TF, VEC, R, WY, NT, ADJ>(in, out, a, cf, t, ft, acc);
+      rows_tile<TS, TF, VEC, R, WY, NT, ADJ, SR>(in, out, a, cf, t, ft, acc);
   }
   if constexpr (ADJ) {
     if (ft != nullptr) block_reduce_vacc<64 * WY>(acc, partials);
@@ -816,13 +861,14 @@ RowsCfg rows_cfg(int bytes, bool adj) {
 
 // Launches the rows kernel for one step (forward: dirs = u, adjoint:
 // dirs = -u).  With dry = true only reports the block count.
-template <typename TS, typename TF, bool ADJ>
+template <typename TS, typename TF, bool ADJ, bool SR = false>
 hipError_t rows_dispatch(const Grid3& g, const TS* in, TS* out, const int dirs[3],
                          const TF* ft, double* partials, int64_t* nblocks_out, hipStream_t s,
-                         bool dry) {
+                         bool dry, uint32_t sr_key = 0) {
   using M = typename St<TS>::M;
   const RowsCfg cfg = rows_cfg((int)sizeof(TS), ADJ);
   MarchArgs a;
+  a.sr_key = sr_key;
   a.n0 = g.n[0];
   a.n1 = g.n[1];
   a.n2 = g.n[2];
@@ -861,7 +907,7 @@ hipError_t rows_dispatch(const Grid3& g, const TS* in, TS* out, const int dirs[3
 #define FDB_ROWS(V, RR, W, NTF)                                                             \
   if constexpr (V <= max_vec<TS>() && V <= max_vec<TF>()) {                                 \
     if (!launched && vec == V && R == RR && WY == W && (cfg.nt != 0) == NTF) {             \
-      hipLaunchKernelGGL((rows_kernel<TS, TF, V, RR, W, NTF, ADJ>), dim3((unsigned)nb),     \
+      hipLaunchKernelGGL((rows_kernel<TS, TF, V, RR, W, NTF, ADJ, SR>), dim3((unsigned)nb), \
                          dim3(64, W), 0, s, in, out, a, cf, ntiles, ft, partials);          \
       launched = true;                                                                      \
     }                                                                                       \
@@ -993,10 +1039,11 @@ hipError_t tb_dispatch(const Grid3& g, const TS* in, TS* out, int K, hipStream_t
   return hipGetLastError();
 }
 
-template <typename TS>
-hipError_t step_typed(const Grid3& g, const void* in, void* out, hipStream_t s) {
-  return rows_dispatch<TS, TS, false>(g, (const TS*)in, (TS*)out, g.up, (const TS*)nullptr,
-                                      nullptr, nullptr, s, false);
+template <typename TS, bool SR = false>
+hipError_t step_typed(const Grid3& g, const void* in, void* out, hipStream_t s,
+                      uint32_t sr_key = 0) {
+  return rows_dispatch<TS, TS, false, SR>(g, (const TS*)in, (TS*)out, g.up, (const TS*)nullptr,
+                                          nullptr, nullptr, s, false, sr_key);
 }
 
 // The transpose step: directions -u; with f_t the velocity-gradient
@@ -1005,7 +1052,7 @@ hipError_t adjoint_dispatch(int lam_dtype, int f_dtype, const Grid3& g, const vo
                             void* lam, const void* f_t, double* partials, int64_t* nb,
                             hipStream_t s, bool dry) {
   const int dirs[3] = {-g.up[0], -g.up[1], -g.up[2]};
-  if (f_t == nullptr) f_dtype = lam_dtype;
+  f_dtype = f_t == nullptr ? lam_dtype : storage_dtype(f_dtype);
 #define FDB_ADJ(TL, TF)                                                                    \
   return rows_dispatch<TL, TF, true>(g, (const TL*)lam_next, (TL*)lam, dirs, (const TF*)f_t, \
                                      partials, nb, s, dry)
@@ -1028,6 +1075,7 @@ size_t dtype_size(int dtype) {
     case FDB_F64: return 8;
     case FDB_F32: return 4;
     case FDB_BF16: return 2;
+    case FDB_BF16_SR: return 2;
     default: return 0;
   }
 }
@@ -1059,11 +1107,17 @@ bool make_grid(int ndims, const uint64_t* n, const double* v, const double* len,
   return true;
 }
 
+uint32_t sr_step_key(uint64_t seed, uint64_t step) {
+  const uint32_t k1 = lowbias32((uint32_t)step + 0x9E3779B9u * (uint32_t)(step >> 32));
+  return lowbias32(k1 ^ (uint32_t)seed ^ ((uint32_t)(seed >> 32) * 0x85EBCA6Bu));
+}
+
 hipError_t launch_advect_pass(int dtype, const Grid3& g, void* const bufs[2], int* cur, int k,
-                              hipStream_t s, int* launches) {
+                              hipStream_t s, int* launches, uint64_t step0, uint64_t seed) {
   if (bufs[0] == bufs[1] || (*cur != 0 && *cur != 1)) return hipErrorInvalidValue;
   hipError_t err = hipSuccess;
   int left = k, n = 0;
+  uint64_t stepno = step0;
   while (left > 0) {
     const void* src = bufs[*cur];
     void* dst = bufs[1 - *cur];
@@ -1075,8 +1129,7 @@ hipError_t launch_advect_pass(int dtype, const Grid3& g, void* const bufs[2], in
         case FDB_F64: err = tb_dispatch<double>(g, (const double*)src, (double*)dst, K, s); break;
         case FDB_F32: err = tb_dispatch<float>(g, (const float*)src, (float*)dst, K, s); break;
         case FDB_BF16:
-          err = tb_dispatch<uint16_t>(g, (const uint16_t*)src, (uint16_t*)dst, K, s);
-          break;
+        case FDB_BF16_SR: err = hipErrorNotSupported; break;  // one-step kernel is faster
         default: return hipErrorInvalidValue;
       }
       if (err == hipSuccess) {
@@ -1090,6 +1143,9 @@ hipError_t launch_advect_pass(int dtype, const Grid3& g, void* const bufs[2], in
         case FDB_F64: err = step_typed<double>(g, src, dst, s); break;
         case FDB_F32: err = step_typed<float>(g, src, dst, s); break;
         case FDB_BF16: err = step_typed<uint16_t>(g, src, dst, s); break;
+        case FDB_BF16_SR:
+          err = step_typed<uint16_t, true>(g, src, dst, s, sr_step_key(seed, stepno));
+          break;
         default: return hipErrorInvalidValue;
       }
       if (err != hipSuccess) return err;
@@ -1097,6 +1153,7 @@ hipError_t launch_advect_pass(int dtype, const Grid3& g, void* const bufs[2], in
     }
     *cur = 1 - *cur;
     left -= done;
+    stepno += (uint64_t)done;
     ++n;
   }
   if (launches) *launches = n;
@@ -1123,6 +1180,7 @@ int64_t adjoint_blocks(int lam_dtype, int f_dtype, const Grid3& g, const void* l
 
 hipError_t launch_sum(int dtype, const void* x, uint64_t n, double* partials, double* out2,
                       hipStream_t s) {
+  dtype = storage_dtype(dtype);
   const dim3 grid(kReduceBlocks), block(256);
   switch (dtype) {
     case FDB_F64: hipLaunchKernelGGL(sum_kernel<double>, grid, block, 0, s, (const double*)x, n, partials); break;
@@ -1138,6 +1196,7 @@ hipError_t launch_sum(int dtype, const void* x, uint64_t n, double* partials, do
 hipError_t launch_misfit(int f_dtype, const void* f, const void* target, uint64_t n,
                          int lam_dtype, void* lam, double* partials, double* out2,
                          hipStream_t s) {
+  f_dtype = storage_dtype(f_dtype);
   const dim3 grid(kReduceBlocks), block(256);
 #define FDB_MF(TF, TL)                                                                      \
   hipLaunchKernelGGL((misfit_kernel<TF, TL>), grid, block, 0, s, (const TF*)f,              \
@@ -1158,6 +1217,7 @@ hipError_t launch_misfit(int f_dtype, const void* f, const void* target, uint64_
 hipError_t launch_init(int dtype, const Grid3& g, const double* len, int kind,
                        const double* params, int nparams, uint64_t seed, void* f,
                        hipStream_t s) {
+  dtype = storage_dtype(dtype);
   InitArgs a = {};
   int r = 0;
   for (int j = 0; j < 3; ++j) {
@@ -1200,6 +1260,8 @@ hipError_t launch_init(int dtype, const Grid3& g, const double* len, int kind,
 }
 
 hipError_t launch_convert(int sd, const void* src, int dd, void* dst, uint64_t n, hipStream_t s) {
+  sd = storage_dtype(sd);
+  dd = storage_dtype(dd);
   const dim3 grid(4096), block(256);
 #define FDB_CV(TA, TB) \
   hipLaunchKernelGGL((convert_kernel<TA, TB>), grid, block, 0, s, (const TA*)src, (TB*)dst, n)
@@ -1216,6 +1278,8 @@ hipError_t launch_convert(int sd, const void* src, int dd, void* dst, uint64_t n
 
 hipError_t launch_compare(int da, const void* a, int db, const void* b, uint64_t n,
                           double* partials, hipStream_t s) {
+  da = storage_dtype(da);
+  db = storage_dtype(db);
   const dim3 grid(kReduceBlocks), block(256);
 #define FDB_CMP(TA, TB) \
   hipLaunchKernelGGL((compare_kernel<TA, TB>), grid, block, 0, s, (const TA*)a, (const TB*)b, n, partials)

@@ -154,7 +154,10 @@ void tape_free(fdb_tape* t) {
 int step_into(fdb_tape* t, void* dst, hipStream_t s) {
   void* const pair[2] = {const_cast<void*>(t->w), dst};
   int cur = 0;
-  TAPE_HIP(fdb::launch_advect_pass(t->f_dtype, t->g, pair, &cur, 1, s, nullptr));
+  // The step's absolute index keys stochastic rounding (FDB_BF16_SR), so a
+  // recomputed state has the same bits as the one the forward produced.
+  TAPE_HIP(fdb::launch_advect_pass(t->f_dtype, t->g, pair, &cur, 1, s, nullptr,
+                                   (uint64_t)t->wstep, 0));
   t->w = dst;
   t->wstep += 1;
   t->fwd_steps += 1;

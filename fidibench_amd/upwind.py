@@ -18,7 +18,7 @@ import numpy as np
 
 from . import _lib as L
 
-_NP = {L.F64: np.float64, L.F32: np.float32, L.BF16: np.uint16}
+_NP = {L.F64: np.float64, L.F32: np.float32, L.BF16: np.uint16, L.BF16_SR: np.uint16}
 
 
 class Upwind:
@@ -108,6 +108,10 @@ class Upwind:
         r = C.c_double()
         L.check(L.lib().fdb_upwind_mass(self._h, C.byref(r)))
         return r.value
+
+    def set_seed(self, seed: int) -> None:
+        """Seed of the stochastic rounding (dtype 'bf16sr'); restarts the step count."""
+        L.check(L.lib().fdb_upwind_set_seed(self._h, int(seed)))
 
     def set_time_block(self, k: int) -> None:
         L.check(L.lib().fdb_upwind_set_time_block(self._h, int(k)))

@@ -33,8 +33,12 @@ extern "C" {
 typedef struct fdb_upwind fdb_upwind_t;
 
 /* Field storage types.  F64 is the reference's type and is bit-exact with
- * it; F32 and BF16 store the field in 4 and 2 bytes and compute in float. */
-enum fdb_dtype { FDB_F64 = 0, FDB_F32 = 1, FDB_BF16 = 2 };
+ * it; F32 and BF16 store the field in 4 and 2 bytes and compute in float.
+ * BF16_SR is bf16 storage whose step results are rounded stochastically
+ * (unbiased; the random bits are a hash of the seed, the step's index and
+ * the cell's flat index, so runs are reproducible).  Host conversions into
+ * BF16 and BF16_SR fields round to nearest even. */
+enum fdb_dtype { FDB_F64 = 0, FDB_F32 = 1, FDB_BF16 = 2, FDB_BF16_SR = 3 };
 
 enum fdb_status {
   FDB_OK = 0,
@@ -111,6 +115,9 @@ int fdb_upwind_mass(fdb_upwind_t* h, double* out);
 /* Steps fused per HBM pass in advect (temporal blocking); 1 = one kernel
  * launch per step.  Results are identical for every k. */
 int fdb_upwind_set_time_block(fdb_upwind_t* h, int k);
+/* Seed of the stochastic rounding of an FDB_BF16_SR handle (default 0);
+ * also restarts the handle's step count, which indexes the random bits. */
+int fdb_upwind_set_seed(fdb_upwind_t* h, uint64_t seed);
 /* advect(), returning the device time of its kernels (ms, HIP events on
  * the handle's stream) and the number of kernel launches. */
 int fdb_upwind_advect_timed(fdb_upwind_t* h, int num_time_steps, double delta_time,

@@ -47,6 +47,10 @@ def lib() -> C.CDLL:
             "orc_advect_f64": (C.c_int, [C.c_int, _u64p, _dp, _dp, C.c_double, C.c_int, _dp]),
             "orc_advect_f32": (C.c_int, [C.c_int, _u64p, _dp, _dp, C.c_double, C.c_int, _fp]),
             "orc_advect_bf16": (C.c_int, [C.c_int, _u64p, _dp, _dp, C.c_double, C.c_int, _u16p]),
+            "orc_advect_bf16_sr": (C.c_int, [C.c_int, _u64p, _dp, _dp, C.c_double, C.c_int,
+                                             C.c_uint64, C.c_uint64, _u16p]),
+            "orc_bf16_sr": (C.c_uint16, [C.c_float, C.c_uint64, C.c_uint32]),
+            "orc_sr_step_key": (C.c_uint32, [C.c_uint64, C.c_uint64]),
             "orc_adjoint_f64": (C.c_int, [C.c_int, _u64p, _dp, _dp, C.c_double, C.c_int, _dp]),
             "orc_misfit_grad_f64": (C.c_int, [C.c_int, _u64p, _dp, _dp, C.c_double, C.c_int,
                                               _dp, _dp, _dp, _dp, _dp]),
@@ -132,6 +136,19 @@ def advect(f0: np.ndarray, v, lengths, dt: float, nsteps: int, dtype: str | None
         raise ValueError(dtype)
     if rc:
         raise RuntimeError(f"oracle advect failed: {rc}")
+    return f
+
+
+def advect_bf16_sr(f0: np.ndarray, v, lengths, dt: float, nsteps: int, seed: int = 0,
+                   step0: int = 0) -> np.ndarray:
+    """bf16 storage (uint16 bit patterns) with stochastically rounded steps."""
+    assert f0.dtype == np.uint16
+    n = f0.shape
+    f = np.ascontiguousarray(f0).copy()
+    rc = lib().orc_advect_bf16_sr(len(n), _u64(n), _d(v), _d(lengths), dt, nsteps, seed, step0,
+                                  _ptr(f, C.c_uint16))
+    if rc:
+        raise RuntimeError(rc)
     return f
 
 

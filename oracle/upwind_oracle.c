@@ -205,6 +205,60 @@ int orc_advect_bf16(int nd, const uint64_t* n, const double* v, const double* le
   return 0;
 }
 
+/* bfloat16 storage with stochastic rounding of every step's result: 16
+ * random bits added below the kept bits of the binary32 value, then
+ * truncated.  The bits are lowbias32 (C. Wellons' integer hash) of the
+ * folded flat cell index xor a per-step key hashed from (seed, step). */
+static uint32_t orc_lowbias32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+uint32_t orc_sr_step_key(uint64_t seed, uint64_t step) {
+  uint32_t k1 = orc_lowbias32((uint32_t)step + 0x9E3779B9u * (uint32_t)(step >> 32));
+  return orc_lowbias32(k1 ^ (uint32_t)seed ^ ((uint32_t)(seed >> 32) * 0x85EBCA6Bu));
+}
+
+uint16_t orc_bf16_sr(float x, uint64_t cell, uint32_t key) {
+  uint32_t b;
+  memcpy(&b, &x, 4);
+  if ((b & 0x7f800000u) == 0x7f800000u)
+    return (uint16_t)((b >> 16) | ((b & 0x007fffffu) ? 0x0040u : 0u));
+  uint32_t r = orc_lowbias32((uint32_t)(cell ^ (cell >> 32)) ^ key) & 0xffffu;
+  return (uint16_t)((b + r) >> 16);
+}
+
+/* As orc_advect_bf16, with step number step0 + s rounded stochastically. */
+int orc_advect_bf16_sr(int nd, const uint64_t* n, const double* v, const double* len,
+                       double dt, int nsteps, uint64_t seed, uint64_t step0, uint16_t* f) {
+  orc_geom g;
+  if (orc_geom_init(&g, nd, n, v, len, dt)) return -1;
+  float cf[ORC_MAXD];
+  for (int j = 0; j < nd; ++j) cf[j] = (float)g.coeff[j];
+  float* old = (float*)malloc(g.ntot * sizeof(float));
+  if (!old) return -2;
+  const int64_t ntot = (int64_t)g.ntot;
+  for (int s = 0; s < nsteps; ++s) {
+    const uint32_t key = orc_sr_step_key(seed, step0 + (uint64_t)s);
+    for (int64_t i = 0; i < ntot; ++i) old[i] = orc_bf16_to_f32(f[i]);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < ntot; ++i) {
+      float fi = old[i];
+      for (int j = 0; j < nd; ++j) {
+        uint64_t up = orc_shift(&g, (uint64_t)i, j, g.up[j]);
+        fi = fi - cf[j] * (old[up] - old[i]);
+      }
+      f[i] = orc_bf16_sr(fi, (uint64_t)i, key);
+    }
+  }
+  free(old);
+  return 0;
+}
+
 /* ------------------------------------------------------------- adjoint --- */
 
 /* One adjoint step lam_next -> lam: the transpose of the forward step

@@ -99,6 +99,32 @@ def test_checkpointing_is_exact(snaps):
     assert np.array_equal(bits(full[2]), bits(part[2]))
 
 
+def test_checkpointing_exact_with_stochastic_rounding():
+    """bf16 storage with stochastic rounding: recomputed states use the same
+    random bits (keyed by the absolute step), so few snapshots give the
+    same gradient bits as storing every state."""
+    from fidibench_amd import _lib as L
+    from fidibench_amd.autograd import Tape
+    shape, v, dt, T = (12, 16, 64), (0.45, -0.3, 0.2), 0.004, 17
+    g = torch.Generator().manual_seed(3)
+    f0 = torch.rand(shape, generator=g, dtype=torch.float32).to(torch.bfloat16).cuda()
+    lamT = torch.rand(shape, generator=g, dtype=torch.float32).cuda()
+    fbytes = f0.numel() * 2
+    outs = []
+    for snaps in (16, 1):
+        tape = Tape(shape, v, (1, 1, 1), dt, T, L.BF16_SR, L.F32,
+                    3 * fbytes + lamT.numel() * 4 + snaps * fbytes)
+        fT = torch.empty_like(f0)
+        tape.forward(f0, fT)
+        lam0 = torch.empty_like(lamT)
+        gv = tape.backward(lamT, lam0)
+        outs.append((fT.view(torch.int16).cpu(), lam0.cpu(), gv, tape.stats()["snapshots"]))
+        tape.close()
+    assert outs[0][3] == 16 and outs[1][3] == 1
+    assert torch.equal(outs[0][0], outs[1][0])
+    assert torch.equal(outs[0][1], outs[1][1]) and outs[0][2] == outs[1][2]
+
+
 def test_tape_forward_steps_follow_binomial_optimum():
     from math import comb
     from fidibench_amd import _lib as L

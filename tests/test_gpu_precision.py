@@ -64,6 +64,48 @@ def test_special_values_match_oracle(oracle, dtype):
     assert np.array_equal(bits(got)[~nan_g], bits(ref)[~nan_r])
 
 
+@pytest.mark.parametrize("seed", [0, 12345])
+def test_bf16_stochastic_rounding_bit_exact_vs_oracle(oracle, seed):
+    """Stochastically rounded bf16 steps (dtype bf16sr) against the
+    oracle's restatement of the same hash, including the step count
+    carried across advect calls."""
+    from fidibench_amd import Upwind
+    for shape, v in [((9, 12, 130), (0.3, -0.45, 0.2)), ((5, 33, 64), (-0.6, 0.1, -0.25)),
+                     ((40, 7), (0.5, -0.3)), ((101,), (-0.7,))]:
+        from fidibench_amd import configs as K
+        dt = K.cfl_dt(shape, v, (1.0,) * len(shape))
+        f0 = oracle.uniform(shape, 0.0, 1.0, seed=4)
+        up = Upwind(v, (1.0,) * len(shape), shape, dtype="bf16sr")
+        up.set_field(f0)
+        up.set_seed(seed)
+        start = up.get_field()
+        up.advect(3, dt)
+        up.advect(4, dt)  # steps 3..6 continue the stream
+        ref = oracle.advect_bf16_sr(start, v, (1.0,) * len(shape), dt, 7, seed=seed, step0=0)
+        assert np.array_equal(up.get_field(), ref), shape
+        up.close()
+
+
+def test_bf16_stochastic_rounding_accuracy():
+    """Gaussian bubble, 192^3 x 200: stochastic rounding removes the
+    stagnation bias of round-to-nearest bf16 storage."""
+    from fidibench_amd import Upwind, configs as K
+    w = K.gauss1024(n=192, steps=200)
+    out = {}
+    for dt_name in ("f64", "bf16", "bf16sr"):
+        up = Upwind(w.velocity, w.lengths, w.num_cells, dtype=dt_name)
+        up.init_field(w.init_kind, w.init_params, seed=w.init_seed)
+        m0 = up.mass()
+        up.advect(w.steps, w.dt)
+        out[dt_name] = (up.get_field("f64"), abs(up.mass() - m0) / abs(m0))
+        up.close()
+    ref = out["f64"][0]
+    e_rne, e_sr = _errors(out["bf16"][0], ref), _errors(out["bf16sr"][0], ref)
+    print("bf16 RNE", e_rne, "drift", out["bf16"][1], "| bf16 SR", e_sr, "drift", out["bf16sr"][1])
+    assert e_sr["l2"] < 0.5 * e_rne["l2"]
+    assert out["bf16sr"][1] < 0.1 * out["bf16"][1]
+
+
 def _errors(x, ref):
     d = x.astype(np.float64) - ref
     return {"max_abs": float(np.abs(d).max()), "l2": float(np.sqrt(np.mean(d * d))),

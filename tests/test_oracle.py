@@ -182,6 +182,41 @@ def test_bf16_rounding_matches_torch_rne(oracle):
     assert (nan & 0x7F80) == 0x7F80 and (nan & 0x7F) != 0
 
 
+def _lowbias32(x):
+    m = 0xFFFFFFFF
+    x ^= x >> 16
+    x = (x * 0x7FEB352D) & m
+    x ^= x >> 15
+    x = (x * 0x846CA68B) & m
+    x ^= x >> 16
+    return x
+
+
+def test_bf16_stochastic_rounding_oracle(oracle):
+    """Each result is one of the two bf16 neighbours of x; bf16-exact values
+    are kept; the mean over many cells is x (unbiased); the random bits are
+    lowbias32 as published (Python restatement)."""
+    L = oracle.lib()
+    for seed, step in [(0, 0), (7, 3), (2 ** 40 + 5, 2 ** 33 + 1)]:
+        k1 = _lowbias32((step + 0x9E3779B9 * (step >> 32)) & 0xFFFFFFFF)
+        key = _lowbias32(k1 ^ (seed & 0xFFFFFFFF) ^ (((seed >> 32) * 0x85EBCA6B) & 0xFFFFFFFF))
+        assert L.orc_sr_step_key(seed, step) == key
+    x = np.float32(1.0 + 2.0 ** -10 + 2.0 ** -12)  # between two bf16 values
+    lo = oracle.f32_to_bf16(np.array([np.float32(1.0)]))[0]
+    hi = lo + 1
+    key = L.orc_sr_step_key(3, 9)
+    outs = np.array([L.orc_bf16_sr(float(x), i, key) for i in range(20000)], dtype=np.uint16)
+    assert set(np.unique(outs)) <= {lo, hi}
+    mean = oracle.bf16_to_f32(outs).astype(np.float64).mean()
+    assert abs(mean - float(x)) < 2e-5  # unbiased: E[round(x)] = x
+    for v in (1.0, -2.5, 0.0, 3.0e38):
+        b = oracle.f32_to_bf16(np.array([v], dtype=np.float32))[0]
+        exact = float(oracle.bf16_to_f32(np.array([b]))[0])
+        assert L.orc_bf16_sr(exact, 12345, key) == b
+    assert L.orc_bf16_sr(float("inf"), 1, key) == 0x7F80
+    assert (L.orc_bf16_sr(float("nan"), 1, key) & 0x7FC0) == 0x7FC0
+
+
 def test_f32_error_vs_f64_is_small(oracle):
     n, v = (24, 24, 24), (0.7, -0.4, 0.2)
     from fidibench_amd import configs as K
