@@ -117,6 +117,38 @@ __device__ inline T dpp_shift(T x) {
 template <typename T> __device__ inline T lane_below(T x) { return dpp_shift<0x138>(x); }  // wave_shr:1
 template <typename T> __device__ inline T lane_above(T x) { return dpp_shift<0x130>(x); }  // wave_shl:1
 
+// The same shifts where the edge lane's result is never used: one DPP move
+// per 32 bits, no copy of the old value (the edge lane reads 0).
+template <int CTRL, typename T>
+__device__ inline T dpp_shift_any(T x) {
+  constexpr int W = sizeof(T) / 4;
+  struct Words { int w[W]; };
+  Words a = __builtin_bit_cast(Words, x), r;
+#pragma unroll
+  for (int i = 0; i < W; ++i) r.w[i] = __builtin_amdgcn_mov_dpp(a.w[i], CTRL, 0xf, 0xf, true);
+  return __builtin_bit_cast(T, r);
+}
+
+// Marks a loaded value as used here, so the compiler waits for its load
+// at this point.  Called on the loads issued before a march loop: without
+// it, the wait for them is placed inside the loop where the loop's first
+// use is, and the merged wait count there also stalls on the next plane's
+// loads issued at the top of each iteration (no prefetch overlap).
+template <typename T>
+__device__ inline void settle(const T& x) {
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8, "one or two VGPRs");
+  asm volatile("" ::"v"(x));
+}
+
+// Workgroup barrier that orders LDS accesses only: global loads issued
+// before it stay in flight (a full __syncthreads() fence would wait for
+// them).
+__device__ inline void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __device__ inline int64_t wrap(int64_t x, int64_t n) {
   return x < 0 ? x + n : (x >= n ? x - n : x);
 }
@@ -528,6 +560,18 @@ __device__ inline void rows_tile(const TS* __restrict__ in, TS* __restrict__ out
   };
   load_plane(p, cur, hal, ed);
   load_f(p, fc);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) {
+      settle(prev[r][v]);
+      settle(cur[r][v]);
+      if constexpr (ADJ) settle(fc[r][v]);
+    }
+    settle(ed[r]);
+  }
+#pragma unroll
+  for (int v = 0; v < VEC; ++v) settle(hal[v]);
 
   for (int64_t k = 0; k < cnt; ++k) {
     M nxt[R][VEC], nhal[VEC], ned[R];
@@ -671,51 +715,74 @@ __global__ __launch_bounds__(64 * WY) void rows_kernel(const TS* __restrict__ in
 
 // ------------------------------------------ temporally blocked forward --
 //
-// K steps per pass over HBM.  Each wave owns an extended tile: 64 lanes
-// along axis 2 (one cell each) x R rows of axis 1 (registers), reaching
-// K cells past its output region on the upwind side of both axes; it
-// outputs (64-K) x (R-K) cells per plane.  The wave marches along axis 0
-// from upwind to downwind and, at every plane, computes levels 1..K of
-// the step in registers; level l of plane p needs level l-1 of plane p
-// (in-register rows, neighbouring lanes) and of plane p-upwind (kept per
-// level in prev[l-1]).  It starts K planes before its chunk so that every
-// level is valid when output begins.  Cells within l of the upwind tile
-// edges hold wrong values at level l, but they never reach an output cell
-// (an output cell's level-K value depends only on cells at most K away
-// upwind).  Each value is produced by the same operations on the same
+// K steps per pass over HBM.  A block of WY waves owns an extended tile:
+// 64 lanes along axis 2 (one cell each) x WY*R rows of axis 1 (R rows in
+// registers per wave), reaching K cells past its output region on the
+// upwind side of both axes; it outputs (64-K) x (WY*R-K) cells per
+// plane.  The block marches along axis 0 from upwind to downwind and, at
+// every plane, computes levels 1..K of the step in registers.  Level l of
+// plane p needs level l-1 of plane p -- rows of the same wave, the
+// neighbouring lane (DPP), and one edge row of the neighbouring wave,
+// handed over through LDS at every level -- and of plane p-upwind (kept
+// per level in prev[l-1]).  It starts K planes before its chunk so that
+// every level is valid when output begins.  Cells within l of the upwind
+// tile edges hold wrong values at level l, but they never reach an output
+// cell (an output cell's level-K value depends only on cells at most K
+// away upwind).  Each value is produced by the same operations on the same
 // inputs as K separate steps, so the result is bit-identical.
+//
+// The upwind signs of axes 1 and 2 are template parameters (LO1: the
+// axis-1 neighbour is row r-1; LO2: the axis-2 neighbour is column c-1),
+// so the plane loop has no direction branches.  Row offsets are uniform
+// per wave (scalar registers); each lane adds one 32-bit column offset.
 struct TbArgs {
   int64_t n0, n1, n2;
-  int u0, u1, u2;
+  int u0;
   int64_t chunk;               // output planes per tile along axis 0
   int64_t tiles2, tiles1, ntiles;
 };
 
-template <typename TS, int R, int K>
-__device__ inline void tb_tile(const TS* __restrict__ in, TS* __restrict__ out,
-                               const TbArgs& a, const Coef<typename St<TS>::M>& cf, int64_t t) {
+// x mod n in [0, n) for any x (n > 0).
+__device__ inline int64_t pmod(int64_t x, int64_t n) {
+  x %= n;
+  return x < 0 ? x + n : x;
+}
+
+template <typename TS, int R, int K, int WY, bool LO1, bool LO2>
+__global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
+                                                     TS* __restrict__ out, TbArgs a,
+                                                     Coef<typename St<TS>::M> cf) {
   using M = typename St<TS>::M;
+  // Edge row handed to the neighbouring wave, double-buffered by the
+  // parity of the running level count (one barrier per level suffices).
+  __shared__ M edge[2][WY][64];
+  int64_t t = blockIdx.x;
+  const int64_t nb = gridDim.x;
+  if ((nb & 7) == 0) t = (t & 7) * (nb >> 3) + (t >> 3);  // XCD-contiguous tiles
+  if (t >= a.ntiles) return;                               // whole block
   const int lane = threadIdx.x;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
   const int64_t t2 = t % a.tiles2;
   const int64_t t1 = (t / a.tiles2) % a.tiles1;
   const int64_t ch = t / (a.tiles2 * a.tiles1);
 
   // Columns: output columns o2 .. o2+63-K; the halo is on the upwind side.
   const int64_t o2 = t2 * (64 - K);
-  const bool lo2 = a.u2 < 0;
-  const int64_t cl = lo2 ? o2 - K + lane : o2 + lane;  // unwrapped column of this lane
-  const int64_t col = wrap(cl, a.n2);
-  const bool col_out = (lo2 ? lane >= K : lane < 64 - K) && cl >= 0 && cl < a.n2;
-  // Rows: output rows o1 .. o1+R-1-K.
-  const int64_t o1 = t1 * (R - K);
-  const bool lo1 = a.u1 < 0;
-  int32_t roff[R];  // in-plane offsets (the dispatch requires n1*n2 < 2^31)
+  const int64_t cl = LO2 ? o2 - K + lane : o2 + lane;  // unwrapped column of this lane
+  const uint32_t col = (uint32_t)pmod(cl, a.n2);
+  const bool col_out = (LO2 ? lane >= K : lane < 64 - K) && cl >= 0 && cl < a.n2;
+  // Rows of this wave: block rows w*R .. w*R+R-1 of the extended tile,
+  // whose output rows are o1 .. o1+WY*R-1-K.
+  const int64_t o1 = t1 * (WY * R - K);
+  const int64_t rb = (LO1 ? o1 - K : o1) + (int64_t)w * R;
+  int64_t roff[R];  // uniform in-plane row offsets
   bool row_out[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const int64_t rl = lo1 ? o1 - K + r : o1 + r;
-    roff[r] = (int32_t)(wrap(rl, a.n1) * a.n2 + col);
-    row_out[r] = (lo1 ? r >= K : r < R - K) && rl >= 0 && rl < a.n1;
+    const int64_t rl = rb + r;
+    const int br = w * R + r;
+    roff[r] = pmod(rl, a.n1) * a.n2;
+    row_out[r] = (LO1 ? br >= K : br < WY * R - K) && rl >= 0 && rl < a.n1;
   }
   // Planes: output p_lo .. p_hi-1; march K planes earlier (upwind side).
   const int64_t plane = a.n1 * a.n2;
@@ -731,73 +798,77 @@ __device__ inline void tb_tile(const TS* __restrict__ in, TS* __restrict__ out,
 #pragma unroll
     for (int r = 0; r < R; ++r) prev[l][r] = M(0);
   M x[R], nx[R];
+  int64_t pw = pmod(first, a.n0);  // wrapped index of the current plane
   {
-    const TS* base = in + wrap(first, a.n0) * plane;
+    const TS* base = in + pw * plane;
 #pragma unroll
-    for (int r = 0; r < R; ++r) x[r] = St<TS>::ld(base[roff[r]]);
+    for (int r = 0; r < R; ++r) x[r] = St<TS>::ld((base + roff[r])[col]);
+#pragma unroll
+    for (int r = 0; r < R; ++r) settle(x[r]);
   }
-  int64_t p = first;
+  // One level, in place: row r from level l to l+1 with axis-1 neighbour n1.
+  auto level_row = [&](int l, int r, M n1) {
+    const M xv = x[r];
+    // The edge lane (63, or 0 with LO2) is a halo column: any value.
+    const M n2 = LO2 ? dpp_shift_any<0x138>(xv) : dpp_shift_any<0x130>(xv);
+    M g = xv;
+    g = g - cf.c[0] * (prev[l][r] - xv);
+    g = g - cf.c[1] * (n1 - xv);
+    g = g - cf.c[2] * (n2 - xv);
+    prev[l][r] = xv;
+    x[r] = St<TS>::ld(St<TS>::st(g));  // each level as stored
+  };
+  int lvl = 0;
   for (int64_t m = 0; m < cnt; ++m) {
+    int64_t pn = pw + step;
+    pn = pn < 0 ? a.n0 - 1 : (pn == a.n0 ? 0 : pn);
     if (m + 1 < cnt) {
-      const TS* base = in + wrap(p + step, a.n0) * plane;
+      const TS* base = in + pn * plane;
 #pragma unroll
-      for (int r = 0; r < R; ++r) nx[r] = St<TS>::ld(base[roff[r]]);
+      for (int r = 0; r < R; ++r) nx[r] = St<TS>::ld((base + roff[r])[col]);
     }
 #pragma unroll
-    for (int l = 0; l < K; ++l) {
-      // Level l+1 from level l, in place.  Rows run against the axis-1
-      // upwind direction so that row r still sees the old row r+u1.
-      if (lo1) {
+    for (int l = 0; l < K; ++l, ++lvl) {
+      // Rows run against the axis-1 upwind direction so that row r still
+      // sees the old row r+u1; the edge row that needs the neighbouring
+      // wave goes last, after the barrier.
+      const int buf = lvl & 1;
+      if constexpr (LO1) {
+        if constexpr (WY > 1) edge[buf][w][lane] = x[R - 1];
 #pragma unroll
-        for (int r = R - 1; r >= 0; --r) {
-          const M xv = x[r];
-          const M n1 = r > 0 ? x[r > 0 ? r - 1 : 0] : xv;
-          const M n2 = lo2 ? lane_below(xv) : lane_above(xv);
-          M g = xv;
-          g = g - cf.c[0] * (prev[l][r] - xv);
-          g = g - cf.c[1] * (n1 - xv);
-          g = g - cf.c[2] * (n2 - xv);
-          prev[l][r] = xv;
-          x[r] = St<TS>::ld(St<TS>::st(g));  // each level as stored (bf16 rounds)
+        for (int r = R - 1; r >= 1; --r) level_row(l, r, x[r - 1]);
+        M n1 = x[0];  // the block's upwind edge: halo row, any value
+        if constexpr (WY > 1) {
+          lds_barrier();
+          const M e = edge[buf][w > 0 ? w - 1 : 0][lane];
+          if (w > 0) n1 = e;
         }
+        level_row(l, 0, n1);
       } else {
+        if constexpr (WY > 1) edge[buf][w][lane] = x[0];
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const M xv = x[r];
-          const M n1 = r < R - 1 ? x[r < R - 1 ? r + 1 : R - 1] : xv;
-          const M n2 = lo2 ? lane_below(xv) : lane_above(xv);
-          M g = xv;
-          g = g - cf.c[0] * (prev[l][r] - xv);
-          g = g - cf.c[1] * (n1 - xv);
-          g = g - cf.c[2] * (n2 - xv);
-          prev[l][r] = xv;
-          x[r] = St<TS>::ld(St<TS>::st(g));  // each level as stored (bf16 rounds)
+        for (int r = 0; r < R - 1; ++r) level_row(l, r, x[r + 1]);
+        M n1 = x[R - 1];
+        if constexpr (WY > 1) {
+          lds_barrier();
+          const M e = edge[buf][w + 1 < WY ? w + 1 : WY - 1][lane];
+          if (w + 1 < WY) n1 = e;
         }
+        level_row(l, R - 1, n1);
       }
     }
     if (m >= K && col_out) {
-      TS* ob = out + wrap(p, a.n0) * plane;
+      TS* ob = out + pw * plane;
 #pragma unroll
       for (int r = 0; r < R; ++r)
-        if (row_out[r]) __builtin_nontemporal_store(St<TS>::st(x[r]), ob + roff[r]);
+        if (row_out[r]) __builtin_nontemporal_store(St<TS>::st(x[r]), (ob + roff[r]) + col);
     }
     if (m + 1 < cnt) {
 #pragma unroll
       for (int r = 0; r < R; ++r) x[r] = nx[r];
     }
-    p += step;
+    pw = pn;
   }
-}
-
-template <typename TS, int R, int K, int WY>
-__global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
-                                                     TS* __restrict__ out, TbArgs a,
-                                                     Coef<typename St<TS>::M> cf) {
-  int64_t b = blockIdx.x;
-  const int64_t nb = gridDim.x;
-  if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);  // XCD-contiguous tiles
-  const int64_t t = b * WY + threadIdx.y;                    // one tile per wave
-  if (t < a.ntiles) tb_tile<TS, R, K>(in, out, a, cf, t);
 }
 
 // ------------------------------------------------------------ dispatch ---
@@ -930,19 +1001,19 @@ hipError_t rows_dispatch(const Grid3& g, const TS* in, TS* out, const int dirs[3
 
 // Temporally blocked pass: K steps in one launch.  Returns
 // hipErrorNotSupported when the shape does not fit (the caller then runs
-// one-step launches): all three axes present, n2 >= 64, n1 >= R, n0 > K.
-// FDB_TB_CFG_<bytes>_<K>="R,WY,WAVES" overrides the tile rows, waves per
-// block and the target number of tiles (waves) per launch.
-// WAVES = 0: choose the chunk count from the kernel's occupancy (below).
+// one-step launches): all three axes present (an absent axis has no term
+// at all, which a fused pass would not reproduce for Inf/NaN values).
+// FDB_TB_CFG_<bytes>_<K>="R,WY,BLOCKS" overrides the rows per wave, the
+// waves per block and the target number of tiles (blocks) per launch;
+// BLOCKS = 0 chooses the chunk count from the kernel's occupancy (below).
 struct TbCfg {
-  int R, WY, waves;
+  int R, WY, blocks;
 };
 
 TbCfg tb_cfg(int bytes, int K) {
-  // 512^3 sweeps: R=24 rows for FP64 at K=4 (one wave per SIMD), 16 rows
-  // otherwise; two waves per block.
-  TbCfg c{16, 2, 0};
-  if (bytes == 8 && K >= 4) c.R = 24;
+  TbCfg c{9, 4, 0};
+  (void)bytes;
+  (void)K;
   char name[48];
   snprintf(name, sizeof(name), "FDB_TB_CFG_%d_%d", bytes, K);
   if (const char* s = getenv(name)) {
@@ -952,19 +1023,19 @@ TbCfg tb_cfg(int bytes, int K) {
   return c;
 }
 
-// Number of axis-0 chunks for `tiles` column tiles: every wave is one
-// tile, the device holds `capacity` waves at once, and a chunk of c
+// Number of axis-0 chunks for `tiles` column tiles: every block is one
+// tile, the device holds `capacity` blocks at once, and a chunk of c
 // planes costs c + K plane visits.  Pick the count that wastes least on
-// the last, partly filled round of waves and on warm-up planes.
+// the last, partly filled round of blocks and on warm-up planes.
 int64_t tb_chunks(int64_t tiles, int64_t n0, int K, int64_t capacity) {
   int64_t best = 1;
   double best_eff = -1.0;
   for (int64_t nch = 1; nch <= 64 && nch <= n0; ++nch) {
     const int64_t chunk = (n0 + nch - 1) / nch;
     const int64_t real = (n0 + chunk - 1) / chunk;
-    const int64_t waves = tiles * real;
-    const int64_t rounds = (waves + capacity - 1) / capacity;
-    const double eff = (double)waves / (double)(rounds * capacity) *
+    const int64_t blocks = tiles * real;
+    const int64_t rounds = (blocks + capacity - 1) / capacity;
+    const double eff = (double)blocks / (double)(rounds * capacity) *
                        (double)n0 / (double)(real * (chunk + K));
     if (eff > best_eff + 1e-9) {
       best_eff = eff;
@@ -993,48 +1064,50 @@ hipError_t tb_dispatch(const Grid3& g, const TS* in, TS* out, int K, hipStream_t
   if (sizeof(TS) < 4) return hipErrorNotSupported;
   const TbCfg cfg = tb_cfg((int)sizeof(TS), K);
   const int R = cfg.R, WY = cfg.WY;
-  if (!(g.act[0] && g.act[1] && g.act[2]) || g.n[2] < 64 || g.n[1] < R || g.n[0] <= K ||
-      R <= K || g.n[1] * g.n[2] >= (int64_t)1 << 31)
-    return hipErrorNotSupported;
+  if (!(g.act[0] && g.act[1] && g.act[2]) || WY * R <= K) return hipErrorNotSupported;
   TbArgs a;
   a.n0 = g.n[0];
   a.n1 = g.n[1];
   a.n2 = g.n[2];
   a.u0 = g.up[0];
-  a.u1 = g.up[1];
-  a.u2 = g.up[2];
   a.tiles2 = (a.n2 + (64 - K) - 1) / (64 - K);
-  a.tiles1 = (a.n1 + (R - K) - 1) / (R - K);
+  a.tiles1 = (a.n1 + (WY * R - K) - 1) / (WY * R - K);
   const int64_t tiles = a.tiles2 * a.tiles1;
+  const bool lo1 = g.up[1] < 0, lo2 = g.up[2] < 0;
   Coef<M> cf;
   for (int j = 0; j < 3; ++j) cf.c[j] = (M)g.c[j];
   bool launched = false;
-#define FDB_TB(RR, KK, W)                                                                   \
-  if (!launched && R == RR && K == KK && WY == W) {                                         \
-    auto kern = tb_kernel<TS, RR, KK, W>;                                                   \
+#define FDB_TB_DIR(RR, KK, W, L1, L2)                                                       \
+  if (!launched && R == RR && K == KK && WY == W && lo1 == L1 && lo2 == L2) {               \
+    auto kern = tb_kernel<TS, RR, KK, W, L1, L2>;                                           \
     int64_t nch;                                                                            \
-    if (cfg.waves > 0) {                                                                    \
-      nch = std::max<int64_t>(1, (cfg.waves + tiles / 2) / tiles);                          \
+    if (cfg.blocks > 0) {                                                                   \
+      nch = std::max<int64_t>(1, (cfg.blocks + tiles / 2) / tiles);                         \
     } else {                                                                                \
       int per_cu = 0;                                                                       \
       if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * W, 0) !=         \
               hipSuccess || per_cu < 1)                                                     \
         per_cu = 1;                                                                         \
-      nch = tb_chunks(tiles, a.n0, KK, (int64_t)per_cu * W * device_cus());                \
+      nch = tb_chunks(tiles, a.n0, KK, (int64_t)per_cu * device_cus());                     \
     }                                                                                       \
     a.chunk = (a.n0 + nch - 1) / nch;                                                       \
     a.ntiles = tiles * ((a.n0 + a.chunk - 1) / a.chunk);                                    \
-    const int64_t nb = (a.ntiles + W - 1) / W;                                              \
-    if (nb > 0x7fffffffLL) return hipErrorInvalidConfiguration;                             \
-    hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(64, W), 0, s, in, out, a, cf);        \
+    if (a.ntiles > 0x7fffffffLL) return hipErrorInvalidConfiguration;                       \
+    hipLaunchKernelGGL(kern, dim3((unsigned)a.ntiles), dim3(64, W), 0, s, in, out, a, cf);  \
     launched = true;                                                                        \
   }
-  FDB_TB(16, 2, 2)
-  FDB_TB(16, 3, 2)
-  FDB_TB(16, 4, 2)
-  FDB_TB(24, 4, 2)
-  FDB_TB(24, 4, 1)
+#define FDB_TB(RR, KK, W)                                                     \
+  FDB_TB_DIR(RR, KK, W, false, false) FDB_TB_DIR(RR, KK, W, false, true)      \
+  FDB_TB_DIR(RR, KK, W, true, false) FDB_TB_DIR(RR, KK, W, true, true)
+  FDB_TB(9, 2, 4)
+  FDB_TB(9, 3, 4)
+  FDB_TB(9, 4, 4)
+  FDB_TB(17, 4, 4)
+  FDB_TB(17, 4, 2)
+  FDB_TB(6, 4, 6)
+  FDB_TB(17, 2, 2)
 #undef FDB_TB
+#undef FDB_TB_DIR
   if (!launched) return hipErrorNotSupported;
   return hipGetLastError();
 }
