@@ -1011,9 +1011,17 @@ struct TbCfg {
 };
 
 TbCfg tb_cfg(int bytes, int K) {
+  // Sweeps at 512^3 and 1024^3 on MI355X, all bit-exact: FP64 runs best
+  // with few rows per wave (occupancy: 5 rows x 16 waves at K=6, 5 x 8 at
+  // K=8), FP32 with fewer redundant halo rows (9 rows x 8 waves at K>=6);
+  // K<=4: 9 rows x 4 waves.
   TbCfg c{9, 4, 0};
-  (void)bytes;
-  (void)K;
+  if (K >= 6) {
+    if (bytes == 8)
+      c = K >= 8 ? TbCfg{5, 8, 0} : TbCfg{5, 16, 0};
+    else
+      c = TbCfg{9, 8, 0};
+  }
   char name[48];
   snprintf(name, sizeof(name), "FDB_TB_CFG_%d_%d", bytes, K);
   if (const char* s = getenv(name)) {
@@ -1102,10 +1110,11 @@ hipError_t tb_dispatch(const Grid3& g, const TS* in, TS* out, int K, hipStream_t
   FDB_TB(9, 2, 4)
   FDB_TB(9, 3, 4)
   FDB_TB(9, 4, 4)
-  FDB_TB(17, 4, 4)
-  FDB_TB(17, 4, 2)
-  FDB_TB(6, 4, 6)
-  FDB_TB(17, 2, 2)
+  FDB_TB(5, 6, 8)
+  FDB_TB(9, 6, 8)
+  FDB_TB(5, 6, 16)
+  FDB_TB(5, 8, 8)
+  FDB_TB(9, 8, 8)
 #undef FDB_TB
 #undef FDB_TB_DIR
   if (!launched) return hipErrorNotSupported;
@@ -1196,7 +1205,7 @@ hipError_t launch_advect_pass(int dtype, const Grid3& g, void* const bufs[2], in
     void* dst = bufs[1 - *cur];
     int done = 0;
     // Temporally blocked pass with the deepest supported K <= left.
-    for (int K : {4, 3, 2}) {
+    for (int K : {8, 6, 4, 3, 2}) {
       if (K > left) continue;
       switch (dtype) {
         case FDB_F64: err = tb_dispatch<double>(g, (const double*)src, (double*)dst, K, s); break;
