@@ -9,7 +9,7 @@
 // "[Cc]heck sum:[ ]*[1|0\\.999]", upwind/cxx/CMakeLists.txt:54-91) run
 // unchanged.  The reference's OpenMP banner lines (upwind.cxx:159-171) are
 // not printed: no OpenMP team runs here, as in a reference build without
-// OpenMP.  Extensions: -precision {f64,f32,bf16}, -timeBlock k.
+// OpenMP.  Extensions: -precision {f64,f32,bf16,bf16sr}, -timeBlock k.
 #include <iostream>
 #include <limits>
 #include <stdexcept>
@@ -28,7 +28,7 @@ int main(int argc, char** argv) {
   args.set("-numSteps", 10, "Number of time steps");
   args.set("-vtk", false, "Write output to VTK file");
   args.set("-std", false, "Print out spread of solution");
-  args.set("-precision", "f64", "Field storage: f64 (bit-exact), f32 or bf16");
+  args.set("-precision", "f64", "Field storage: f64 (bit-exact), f32, bf16 or bf16sr (stochastic rounding)");
   args.set("-timeBlock", 1, "Time steps fused per pass over the field");
   args.setFooter("\nfidibench-amd: upwind advection on the GPU\n");
 
@@ -47,8 +47,9 @@ int main(int argc, char** argv) {
   int dtype = FDB_F64;
   if (prec == "f32") dtype = FDB_F32;
   else if (prec == "bf16") dtype = FDB_BF16;
+  else if (prec == "bf16sr") dtype = FDB_BF16_SR;
   else if (prec != "f64") {
-    std::cerr << "ERROR: -precision must be f64, f32 or bf16\n";
+    std::cerr << "ERROR: -precision must be f64, f32, bf16 or bf16sr\n";
     return 1;
   }
 

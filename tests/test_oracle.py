@@ -217,6 +217,36 @@ def test_bf16_stochastic_rounding_oracle(oracle):
     assert (L.orc_bf16_sr(float("nan"), 1, key) & 0x7FC0) == 0x7FC0
 
 
+def test_bf16_sr_advect_oracle(oracle):
+    """Whole stochastically rounded steps.  On dyadic data where one step's
+    every intermediate and result is exact in bf16, nothing is dropped, so
+    the SR step equals the round-to-nearest step and the FP64 step bit for
+    bit (a wrong index, term or rounding position breaks the equality).  On
+    generic data two seeds differ, one seed reproduces, and a run split in
+    two calls (step0 carried) equals one call."""
+    rng = np.random.default_rng(5)
+    shape = (6, 8, 16)
+    ints = rng.integers(0, 9, size=shape).astype(np.float64) / 8.0
+    b0 = oracle.f32_to_bf16(ints.astype(np.float32)).reshape(shape)
+    # dx = 1/n; c_j = dt*|v_j|*n_j = 1/4, 1/8, 1/8 with dt = 1/64 (lengths 1).
+    v = (16.0 / 6, -8.0 / 8, 8.0 / 16)
+    dt = 1.0 / 64
+    sr = oracle.advect_bf16_sr(b0, v, (1, 1, 1), dt, 1, seed=11)
+    rne = oracle.advect(b0, v, (1, 1, 1), dt, 1)
+    f64 = oracle.advect(ints, v, (1, 1, 1), dt, 1)
+    assert not np.array_equal(f64, ints)  # the step moved the field
+    assert np.array_equal(sr, rne)
+    assert np.array_equal(oracle.bf16_to_f32(sr).astype(np.float64).reshape(shape), f64)
+    g0 = oracle.f32_to_bf16(rng.random(shape, dtype=np.float32)).reshape(shape)
+    a = oracle.advect_bf16_sr(g0, (0.3, -0.2, 0.4), (1, 1, 1), 0.01, 5, seed=1)
+    b = oracle.advect_bf16_sr(g0, (0.3, -0.2, 0.4), (1, 1, 1), 0.01, 5, seed=2)
+    c = oracle.advect_bf16_sr(g0, (0.3, -0.2, 0.4), (1, 1, 1), 0.01, 5, seed=1)
+    assert not np.array_equal(a, b) and np.array_equal(a, c)
+    h = oracle.advect_bf16_sr(g0, (0.3, -0.2, 0.4), (1, 1, 1), 0.01, 2, seed=1)
+    h = oracle.advect_bf16_sr(h, (0.3, -0.2, 0.4), (1, 1, 1), 0.01, 3, seed=1, step0=2)
+    assert np.array_equal(a, h)
+
+
 def test_f32_error_vs_f64_is_small(oracle):
     n, v = (24, 24, 24), (0.7, -0.4, 0.2)
     from fidibench_amd import configs as K
