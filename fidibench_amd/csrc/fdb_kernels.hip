@@ -892,6 +892,55 @@ struct RowsCfg {
 
 constexpr int64_t kTargetBlocks = 1024;
 
+int device_cus() {
+  static int cus = [] {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      n = 256;
+    return n > 0 ? n : 256;
+  }();
+  return cus;
+}
+
+// Blocks of `kern` (with `threads` threads) the device holds at once.
+template <typename Kern>
+int64_t resident_blocks(Kern kern, int threads) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, 0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  return (int64_t)per_cu * device_cus();
+}
+
+// Number of axis-0 chunks, between lo and hi, for `tiles` tiles per chunk:
+// every block is one tile, the device holds `capacity` blocks at once, and
+// a chunk of c planes costs c + extra plane visits (planes read before the
+// first output plane).  Picks the count that wastes least on the last,
+// partly filled round of blocks and on the extra planes.  With
+// fill = false a single, partly filled round counts as full (a kernel
+// that saturates HBM with fewer resident blocks); with fill = true it
+// counts by its fill (a kernel whose rate scales with resident waves).
+int64_t march_chunks(int64_t tiles, int64_t n0, int extra, int64_t capacity, int64_t lo,
+                     int64_t hi, bool fill) {
+  int64_t best = 1;
+  double best_eff = -1.0;
+  for (int64_t nch = std::max<int64_t>(1, lo); nch <= hi && nch <= n0; ++nch) {
+    const int64_t chunk = (n0 + nch - 1) / nch;
+    const int64_t real = (n0 + chunk - 1) / chunk;
+    const int64_t blocks = tiles * real;
+    const int64_t rounds = (blocks + capacity - 1) / capacity;
+    const double used =
+        (!fill && rounds == 1) ? 1.0 : (double)blocks / (double)(rounds * capacity);
+    const double eff = used * (double)n0 / (double)(real * (chunk + extra));
+    if (eff > best_eff + 1e-9) {
+      best_eff = eff;
+      best = real;
+    }
+  }
+  return best;
+}
+
 RowsCfg parse_cfg(const char* s, RowsCfg c) {
   int r, w, ch, nt, gr = 0;
   if (s && sscanf(s, "%d,%d,%d,%d,%d", &r, &w, &ch, &nt, &gr) >= 4) {
@@ -904,30 +953,48 @@ RowsCfg parse_cfg(const char* s, RowsCfg c) {
   return c;
 }
 
-RowsCfg rows_cfg(int bytes, bool adj) {
-  static RowsCfg cfgs[2][9];
-  static bool init[2][9] = {};
-  if (!init[adj][bytes]) {
+// bytes: storage of the stepped field (lam for the adjoint); fbytes: the
+// adjoint's f_t storage (0: none).  FDB_ADJ_CFG_<bytes>_<fbytes> overrides
+// one adjoint pairing, FDB_ADJ_CFG_<bytes> all pairings of a lam type.
+RowsCfg rows_cfg(int bytes, bool adj, int fbytes = 0) {
+  static RowsCfg cfgs[2][9][9];
+  static bool init[2][9][9] = {};
+  if (!adj) fbytes = 0;
+  if (!init[adj][bytes][fbytes]) {
     RowsCfg c;
     // 512^3 sweeps: FP32 R=4 x 2 waves (715 G cells/s, 2.0x FP64), BF16
-    // R=2 x 4 waves; the adjoint carries f_t as well: R=8 x 1 wave.
+    // R=2 x 4 waves; the adjoint carries f_t as well: R=8 x 1 wave, and
+    // R=4 x 2 waves for an FP32 lam with a bf16 f_t (R=8 spills there, and
+    // every spilled f_t load waits for its data at once).
     if (adj) {
       c.R = 8;
       c.WY = 1;
+      if (bytes == 4 && fbytes == 2) {
+        c.R = 4;
+        c.WY = 2;
+      }
     } else if (bytes == 4) {
       c.R = 4;
     } else if (bytes == 2) {
       c.R = 2;
       c.WY = 4;
     }
-    char name[32];
-    snprintf(name, sizeof(name), adj ? "FDB_ADJ_CFG_%d" : "FDB_ROWS_CFG_%d", bytes);
-    const char* s = getenv(name);
-    if (!s && !adj) s = getenv("FDB_ROWS_CFG");
-    cfgs[adj][bytes] = parse_cfg(s, c);
-    init[adj][bytes] = true;
+    char name[40];
+    const char* s = nullptr;
+    if (adj) {
+      snprintf(name, sizeof(name), "FDB_ADJ_CFG_%d_%d", bytes, fbytes);
+      s = getenv(name);
+      snprintf(name, sizeof(name), "FDB_ADJ_CFG_%d", bytes);
+      if (!s) s = getenv(name);
+    } else {
+      snprintf(name, sizeof(name), "FDB_ROWS_CFG_%d", bytes);
+      s = getenv(name);
+      if (!s) s = getenv("FDB_ROWS_CFG");
+    }
+    cfgs[adj][bytes][fbytes] = parse_cfg(s, c);
+    init[adj][bytes][fbytes] = true;
   }
-  return cfgs[adj][bytes];
+  return cfgs[adj][bytes][fbytes];
 }
 
 // Launches the rows kernel for one step (forward: dirs = u, adjoint:
@@ -937,7 +1004,7 @@ hipError_t rows_dispatch(const Grid3& g, const TS* in, TS* out, const int dirs[3
                          const TF* ft, double* partials, int64_t* nblocks_out, hipStream_t s,
                          bool dry, uint32_t sr_key = 0) {
   using M = typename St<TS>::M;
-  const RowsCfg cfg = rows_cfg((int)sizeof(TS), ADJ);
+  const RowsCfg cfg = rows_cfg((int)sizeof(TS), ADJ, ft ? (int)sizeof(TF) : 0);
   MarchArgs a;
   a.sr_key = sr_key;
   a.n0 = g.n[0];
@@ -961,27 +1028,42 @@ hipError_t rows_dispatch(const Grid3& g, const TS* in, TS* out, const int dirs[3
   a.tiles2 = (a.n2 + 64 * vec - 1) / (64 * vec);
   a.tiles1 = (a.n1 + (int64_t)WY * R - 1) / ((int64_t)WY * R);
   const int64_t tiles = a.tiles2 * a.tiles1;
-  // Axis-0 chunk: an explicit one, or enough equal chunks for about
-  // kTargetBlocks tiles (4 per CU), each block taking one tile.
-  int64_t chunk = cfg.chunk;
-  if (chunk <= 0) {
-    const int64_t nch = std::max<int64_t>(1, (kTargetBlocks + tiles / 2) / tiles);
-    chunk = (a.n0 + nch - 1) / nch;
-  }
-  a.chunk = chunk;
-  const int64_t ntiles = tiles * ((a.n0 + chunk - 1) / chunk);
-  const int64_t nb = (cfg.grid > 0 && cfg.grid < ntiles) ? cfg.grid : ntiles;
-  if (nblocks_out) *nblocks_out = nb;
-  if (dry) return hipSuccess;
-  if (nb <= 0 || nb > 0x7fffffffLL) return hipErrorInvalidConfiguration;
+  // Axis-0 chunk: an explicit one, or equal chunks for about kTargetBlocks
+  // tiles (4 per CU) and up to 4x more, whichever count fills the last
+  // round of resident blocks best (occupancy of the kernel launched); each
+  // block takes one tile.
+  hipError_t result = hipErrorInvalidValue;
+  auto go = [&](auto kern, int W) {
+    int64_t chunk = cfg.chunk;
+    if (chunk <= 0) {
+      const int64_t lo = std::max<int64_t>(1, (kTargetBlocks + tiles / 2) / tiles);
+      const int64_t nch =
+          march_chunks(tiles, a.n0, 1, resident_blocks(kern, 64 * W), lo, 4 * lo, false);
+      chunk = (a.n0 + nch - 1) / nch;
+    }
+    a.chunk = chunk;
+    const int64_t ntiles = tiles * ((a.n0 + chunk - 1) / chunk);
+    const int64_t nb = (cfg.grid > 0 && cfg.grid < ntiles) ? cfg.grid : ntiles;
+    if (nblocks_out) *nblocks_out = nb;
+    if (dry) {
+      result = hipSuccess;
+      return;
+    }
+    if (nb <= 0 || nb > 0x7fffffffLL) {
+      result = hipErrorInvalidConfiguration;
+      return;
+    }
+    hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(64, W), 0, s, in, out, a, cf, ntiles, ft,
+                       partials);
+    result = hipGetLastError();
+  };
   bool launched = false;
-#define FDB_ROWS(V, RR, W, NTF)                                                             \
-  if constexpr (V <= max_vec<TS>() && V <= max_vec<TF>()) {                                 \
-    if (!launched && vec == V && R == RR && WY == W && (cfg.nt != 0) == NTF) {             \
-      hipLaunchKernelGGL((rows_kernel<TS, TF, V, RR, W, NTF, ADJ, SR>), dim3((unsigned)nb), \
-                         dim3(64, W), 0, s, in, out, a, cf, ntiles, ft, partials);          \
-      launched = true;                                                                      \
-    }                                                                                       \
+#define FDB_ROWS(V, RR, W, NTF)                                                 \
+  if constexpr (V <= max_vec<TS>() && V <= max_vec<TF>()) {                     \
+    if (!launched && vec == V && R == RR && WY == W && (cfg.nt != 0) == NTF) { \
+      go(rows_kernel<TS, TF, V, RR, W, NTF, ADJ, SR>, W);                       \
+      launched = true;                                                          \
+    }                                                                           \
   }
 #define FDB_ROWS_V(RR, W, NTF) \
   FDB_ROWS(8, RR, W, NTF) FDB_ROWS(4, RR, W, NTF) FDB_ROWS(2, RR, W, NTF) FDB_ROWS(1, RR, W, NTF)
@@ -995,8 +1077,7 @@ hipError_t rows_dispatch(const Grid3& g, const TS* in, TS* out, const int dirs[3
   FDB_ROWS_V(1, 8, true)
 #undef FDB_ROWS_V
 #undef FDB_ROWS
-  if (!launched) return hipErrorInvalidValue;
-  return hipGetLastError();
+  return launched ? result : hipErrorInvalidValue;
 }
 
 // Temporally blocked pass: K steps in one launch.  Returns
@@ -1031,39 +1112,6 @@ TbCfg tb_cfg(int bytes, int K) {
   return c;
 }
 
-// Number of axis-0 chunks for `tiles` column tiles: every block is one
-// tile, the device holds `capacity` blocks at once, and a chunk of c
-// planes costs c + K plane visits.  Pick the count that wastes least on
-// the last, partly filled round of blocks and on warm-up planes.
-int64_t tb_chunks(int64_t tiles, int64_t n0, int K, int64_t capacity) {
-  int64_t best = 1;
-  double best_eff = -1.0;
-  for (int64_t nch = 1; nch <= 64 && nch <= n0; ++nch) {
-    const int64_t chunk = (n0 + nch - 1) / nch;
-    const int64_t real = (n0 + chunk - 1) / chunk;
-    const int64_t blocks = tiles * real;
-    const int64_t rounds = (blocks + capacity - 1) / capacity;
-    const double eff = (double)blocks / (double)(rounds * capacity) *
-                       (double)n0 / (double)(real * (chunk + K));
-    if (eff > best_eff + 1e-9) {
-      best_eff = eff;
-      best = real;
-    }
-  }
-  return best;
-}
-
-int device_cus() {
-  static int cus = [] {
-    int dev = 0, n = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-      n = 256;
-    return n > 0 ? n : 256;
-  }();
-  return cus;
-}
-
 template <typename TS>
 hipError_t tb_dispatch(const Grid3& g, const TS* in, TS* out, int K, hipStream_t s) {
   using M = typename St<TS>::M;
@@ -1092,11 +1140,7 @@ hipError_t tb_dispatch(const Grid3& g, const TS* in, TS* out, int K, hipStream_t
     if (cfg.blocks > 0) {                                                                   \
       nch = std::max<int64_t>(1, (cfg.blocks + tiles / 2) / tiles);                         \
     } else {                                                                                \
-      int per_cu = 0;                                                                       \
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * W, 0) !=         \
-              hipSuccess || per_cu < 1)                                                     \
-        per_cu = 1;                                                                         \
-      nch = tb_chunks(tiles, a.n0, KK, (int64_t)per_cu * device_cus());                     \
+      nch = march_chunks(tiles, a.n0, KK, resident_blocks(kern, 64 * W), 1, 64, true);      \
     }                                                                                       \
     a.chunk = (a.n0 + nch - 1) / nch;                                                       \
     a.ntiles = tiles * ((a.n0 + a.chunk - 1) / a.chunk);                                    \
@@ -1187,6 +1231,16 @@ bool make_grid(int ndims, const uint64_t* n, const double* v, const double* len,
     g->ntot *= g->n[j];
   }
   return true;
+}
+
+hipError_t launch_fused_pass(int dtype, const Grid3& g, const void* src, void* dst, int k,
+                             hipStream_t s) {
+  if (k < 2 || src == dst) return hipErrorNotSupported;
+  switch (dtype) {
+    case FDB_F64: return tb_dispatch<double>(g, (const double*)src, (double*)dst, k, s);
+    case FDB_F32: return tb_dispatch<float>(g, (const float*)src, (float*)dst, k, s);
+    default: return hipErrorNotSupported;
+  }
 }
 
 uint32_t sr_step_key(uint64_t seed, uint64_t step) {

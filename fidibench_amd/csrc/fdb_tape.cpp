@@ -168,11 +168,32 @@ int step_into(fdb_tape* t, void* dst, hipStream_t s) {
 void* free_scratch(fdb_tape* t) { return t->w == t->scratch[0] ? t->scratch[1] : t->scratch[0]; }
 
 // Advance the working state by a steps; the last one lands in `last_dst`
-// when given (the slot of a snapshot about to be stored).
+// when given (the slot of a snapshot about to be stored).  The states in
+// between are not kept, so runs of steps go through fused multi-step
+// launches where the storage type and shape allow (same bits as single
+// steps); the working state's buffer is only read.
 int advance(fdb_tape* t, int a, void* last_dst, hipStream_t s) {
-  for (int i = 0; i < a; ++i) {
-    void* dst = (i == a - 1 && last_dst) ? last_dst : free_scratch(t);
-    if (int rc = step_into(t, dst, s)) return rc;
+  int left = a;
+  while (left > 0) {
+    int done = 0;
+    for (int k : {8, 6, 4, 3, 2}) {
+      if (k > left || k > t->time_block) continue;
+      void* dst = (k == left && last_dst) ? last_dst : free_scratch(t);
+      const hipError_t e = fdb::launch_fused_pass(t->f_dtype, t->g, t->w, dst, k, s);
+      if (e == hipErrorNotSupported) continue;
+      TAPE_HIP(e);
+      t->w = dst;
+      t->wstep += k;
+      t->fwd_steps += k;
+      done = k;
+      break;
+    }
+    if (done == 0) {
+      void* dst = (left == 1 && last_dst) ? last_dst : free_scratch(t);
+      if (int rc = step_into(t, dst, s)) return rc;
+      done = 1;
+    }
+    left -= done;
   }
   return FDB_OK;
 }
