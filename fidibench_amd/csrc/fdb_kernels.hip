@@ -51,9 +51,11 @@ template <> struct St<uint16_t> {
 // Stochastic rounding of step results for bf16 storage (FDB_BF16_SR): 16
 // random bits are added below the kept bits of the binary32 value, then
 // truncated, so x rounds up with probability equal to the discarded
-// fraction (unbiased).  The bits are lowbias32 (C. Wellons' 32-bit integer
-// hash) of the cell's flat index mixed with a per-step key made on the
-// host from the seed and the step's index; Inf stays Inf, NaN stays quiet.
+// fraction (unbiased).  The bits are one half of lowbias32 (C. Wellons'
+// 32-bit integer hash) of the index of the cell's pair (flat index / 2)
+// mixed with a per-step key made on the host from the seed and the step's
+// index: the low half for an even cell, the high half for an odd one (one
+// hash per two cells).  Inf stays Inf, NaN stays quiet.
 __host__ __device__ inline uint32_t lowbias32(uint32_t x) {
   x ^= x >> 16;
   x *= 0x7feb352du;
@@ -63,12 +65,21 @@ __host__ __device__ inline uint32_t lowbias32(uint32_t x) {
   return x;
 }
 
-__device__ inline uint16_t bf16_sr(float x, uint64_t cell, uint32_t key) {
+__device__ inline uint32_t sr_pair_hash(uint64_t pair, uint32_t key) {
+  return lowbias32((uint32_t)(pair ^ (pair >> 32)) ^ key);
+}
+
+// x rounded with 16 random bits r (r < 2^16).
+__device__ inline uint16_t bf16_sr_bits(float x, uint32_t r) {
   const uint32_t b = __builtin_bit_cast(uint32_t, x);
   if ((b & 0x7f800000u) == 0x7f800000u)
     return (uint16_t)((b >> 16) | ((b & 0x007fffffu) ? 0x0040u : 0u));
-  const uint32_t r = lowbias32((uint32_t)(cell ^ (cell >> 32)) ^ key) & 0xffffu;
   return (uint16_t)((b + r) >> 16);
+}
+
+__device__ inline uint16_t bf16_sr(float x, uint64_t cell, uint32_t key) {
+  const uint32_t h = sr_pair_hash(cell >> 1, key);
+  return bf16_sr_bits(x, (cell & 1) ? (h >> 16) : (h & 0xffffu));
 }
 
 typedef float f32x2 __attribute__((ext_vector_type(2)));
@@ -459,8 +470,20 @@ template <int VEC, bool NT>
 __device__ inline void store_pack_sr(uint16_t* p, const float (&in)[VEC], uint64_t cell,
                                      uint32_t key) {
   Pack<uint16_t, VEC> pk;
+  if constexpr (VEC % 2 == 0) {
+    // cell is even here (n2 % VEC == 0 and the pack starts a VEC group):
+    // one hash serves the pair (cell + v, cell + v + 1).
+    const uint64_t pair0 = cell >> 1;
 #pragma unroll
-  for (int v = 0; v < VEC; ++v) pk.v[v] = bf16_sr(in[v], cell + v, key);
+    for (int v = 0; v < VEC; v += 2) {
+      const uint32_t h = sr_pair_hash(pair0 + (uint64_t)(v / 2), key);
+      pk.v[v] = bf16_sr_bits(in[v], h & 0xffffu);
+      pk.v[v + 1] = bf16_sr_bits(in[v + 1], h >> 16);
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) pk.v[v] = bf16_sr(in[v], cell + v, key);
+  }
   using R = typename RawT<sizeof(Pack<uint16_t, VEC>)>::T;
   const R raw = __builtin_bit_cast(R, pk);
   if constexpr (NT)
@@ -963,15 +986,17 @@ RowsCfg rows_cfg(int bytes, bool adj, int fbytes = 0) {
   if (!init[adj][bytes][fbytes]) {
     RowsCfg c;
     // 512^3 sweeps: FP32 R=4 x 2 waves (715 G cells/s, 2.0x FP64), BF16
-    // R=2 x 4 waves; the adjoint carries f_t as well: R=8 x 1 wave, and
-    // R=4 x 2 waves for an FP32 lam with a bf16 f_t (R=8 spills there, and
-    // every spilled f_t load waits for its data at once).
+    // R=2 x 4 waves.  The adjoint carries f_t as well (1024x1024x768
+    // sweeps): R=8 x 1 wave (FP32 lam and f_t, 5.3 TB/s), R=8 x 2 (FP64,
+    // 5.3 TB/s), R=4 x 4 for an FP32 lam with a bf16 f_t (4.9 TB/s; R=8
+    // spills there, and every spilled f_t load waits for its data at once).
     if (adj) {
       c.R = 8;
       c.WY = 1;
+      if (bytes == 8 && fbytes == 8) c.WY = 2;
       if (bytes == 4 && fbytes == 2) {
         c.R = 4;
-        c.WY = 2;
+        c.WY = 4;
       }
     } else if (bytes == 4) {
       c.R = 4;

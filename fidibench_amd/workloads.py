@@ -39,7 +39,7 @@ def init_field(w: Workload, dtype: int, params=None, seed=None) -> torch.Tensor:
     return f
 
 
-def advect(f0: torch.Tensor, velocity, w: Workload, nsteps: int, time_block: int = 4):
+def advect(f0: torch.Tensor, velocity, w: Workload, nsteps: int, time_block: int = 8):
     """nsteps plain forward steps (no tape); returns a new tensor."""
     a = f0.clone()
     b = torch.empty_like(a)

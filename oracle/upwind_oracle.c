@@ -207,8 +207,10 @@ int orc_advect_bf16(int nd, const uint64_t* n, const double* v, const double* le
 
 /* bfloat16 storage with stochastic rounding of every step's result: 16
  * random bits added below the kept bits of the binary32 value, then
- * truncated.  The bits are lowbias32 (C. Wellons' integer hash) of the
- * folded flat cell index xor a per-step key hashed from (seed, step). */
+ * truncated.  The bits come from lowbias32 (C. Wellons' integer hash) of
+ * the folded index of the cell's pair (cell / 2) xor a per-step key hashed
+ * from (seed, step): the low half for an even cell, the high half for an
+ * odd one. */
 static uint32_t orc_lowbias32(uint32_t x) {
   x ^= x >> 16;
   x *= 0x7feb352du;
@@ -223,13 +225,18 @@ uint32_t orc_sr_step_key(uint64_t seed, uint64_t step) {
   return orc_lowbias32(k1 ^ (uint32_t)seed ^ ((uint32_t)(seed >> 32) * 0x85EBCA6Bu));
 }
 
+uint32_t orc_sr_bits(uint64_t cell, uint32_t key) {
+  const uint64_t pr = cell >> 1;
+  const uint32_t h = orc_lowbias32((uint32_t)(pr ^ (pr >> 32)) ^ key);
+  return (cell & 1u) ? (h >> 16) : (h & 0xffffu);
+}
+
 uint16_t orc_bf16_sr(float x, uint64_t cell, uint32_t key) {
   uint32_t b;
   memcpy(&b, &x, 4);
   if ((b & 0x7f800000u) == 0x7f800000u)
     return (uint16_t)((b >> 16) | ((b & 0x007fffffu) ? 0x0040u : 0u));
-  uint32_t r = orc_lowbias32((uint32_t)(cell ^ (cell >> 32)) ^ key) & 0xffffu;
-  return (uint16_t)((b + r) >> 16);
+  return (uint16_t)((b + orc_sr_bits(cell, key)) >> 16);
 }
 
 /* As orc_advect_bf16, with step number step0 + s rounded stochastically. */

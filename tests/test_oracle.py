@@ -215,6 +215,14 @@ def test_bf16_stochastic_rounding_oracle(oracle):
         assert L.orc_bf16_sr(exact, 12345, key) == b
     assert L.orc_bf16_sr(float("inf"), 1, key) == 0x7F80
     assert (L.orc_bf16_sr(float("nan"), 1, key) & 0x7FC0) == 0x7FC0
+    # Bit layout: one hash per cell pair, low half for the even cell.
+    for cell in (0, 1, 2, 3, 12345, 12346, 2 ** 33 + 7, 2 ** 40 + 2):
+        pr = cell >> 1
+        h = _lowbias32(((pr ^ (pr >> 32)) & 0xFFFFFFFF) ^ key)
+        r = (h >> 16) if cell & 1 else (h & 0xFFFF)
+        assert L.orc_sr_bits(cell, key) == r
+        b = int(np.float32(1.3).view(np.uint32))
+        assert L.orc_bf16_sr(1.3, cell, key) == ((b + r) >> 16) & 0xFFFF
 
 
 def test_bf16_sr_advect_oracle(oracle):
