@@ -830,17 +830,6 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
     for (int r = 0; r < R; ++r) settle(x[r]);
   }
   // One level, in place: row r from level l to l+1 with axis-1 neighbour n1.
-  auto level_row = [&](int l, int r, M n1) {
-    const M xv = x[r];
-    // The edge lane (63, or 0 with LO2) is a halo column: any value.
-    const M n2 = LO2 ? dpp_shift_any<0x138>(xv) : dpp_shift_any<0x130>(xv);
-    M g = xv;
-    g = g - cf.c[0] * (prev[l][r] - xv);
-    g = g - cf.c[1] * (n1 - xv);
-    g = g - cf.c[2] * (n2 - xv);
-    prev[l][r] = xv;
-    x[r] = St<TS>::ld(St<TS>::st(g));  // each level as stored
-  };
   int lvl = 0;
   for (int64_t m = 0; m < cnt; ++m) {
     int64_t pn = pw + step;
@@ -852,32 +841,45 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
     }
 #pragma unroll
     for (int l = 0; l < K; ++l, ++lvl) {
-      // Rows run against the axis-1 upwind direction so that row r still
-      // sees the old row r+u1; the edge row that needs the neighbouring
-      // wave goes last, after the barrier.
+      // Level l+1 from level l in three phases over all rows -- the three
+      // differences, the three products, then g = x - p0 - p1 - p2 -- so
+      // independent rows interleave (each cell still sees the same
+      // operations in the same order).  The row that needs the
+      // neighbouring wave's edge row (row 0 with LO1, else row R-1) gets
+      // its axis-1 difference after the barrier.
       const int buf = lvl & 1;
-      if constexpr (LO1) {
-        if constexpr (WY > 1) edge[buf][w][lane] = x[R - 1];
+      constexpr int er = LO1 ? 0 : R - 1;  // the block-edge row of this wave
+      if constexpr (WY > 1) edge[buf][w][lane] = x[LO1 ? R - 1 : 0];
+      M d0[R], d1[R], d2[R];
 #pragma unroll
-        for (int r = R - 1; r >= 1; --r) level_row(l, r, x[r - 1]);
-        M n1 = x[0];  // the block's upwind edge: halo row, any value
-        if constexpr (WY > 1) {
-          lds_barrier();
-          const M e = edge[buf][w > 0 ? w - 1 : 0][lane];
-          if (w > 0) n1 = e;
-        }
-        level_row(l, 0, n1);
-      } else {
-        if constexpr (WY > 1) edge[buf][w][lane] = x[0];
+      for (int r = 0; r < R; ++r) {
+        // The edge lane (63, or 0 with LO2) is a halo column: any value.
+        const M n2 = LO2 ? dpp_shift_any<0x138>(x[r]) : dpp_shift_any<0x130>(x[r]);
+        d0[r] = prev[l][r] - x[r];
+        d2[r] = n2 - x[r];
+        if (r != er) d1[r] = x[LO1 ? r - 1 : r + 1] - x[r];
+      }
+      M n1 = x[er];  // the block's upwind edge: halo row, any value
+      if constexpr (WY > 1) {
+        lds_barrier();
+        const M e = edge[buf][LO1 ? (w > 0 ? w - 1 : 0) : (w + 1 < WY ? w + 1 : WY - 1)][lane];
+        if (LO1 ? w > 0 : w + 1 < WY) n1 = e;
+      }
+      d1[er] = n1 - x[er];
 #pragma unroll
-        for (int r = 0; r < R - 1; ++r) level_row(l, r, x[r + 1]);
-        M n1 = x[R - 1];
-        if constexpr (WY > 1) {
-          lds_barrier();
-          const M e = edge[buf][w + 1 < WY ? w + 1 : WY - 1][lane];
-          if (w + 1 < WY) n1 = e;
-        }
-        level_row(l, R - 1, n1);
+      for (int r = 0; r < R; ++r) {
+        d0[r] = cf.c[0] * d0[r];
+        d1[r] = cf.c[1] * d1[r];
+        d2[r] = cf.c[2] * d2[r];
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        M g = x[r];
+        g = g - d0[r];
+        g = g - d1[r];
+        g = g - d2[r];
+        prev[l][r] = x[r];
+        x[r] = St<TS>::ld(St<TS>::st(g));  // each level as stored
       }
     }
     if (m >= K && col_out) {
@@ -1117,17 +1119,11 @@ struct TbCfg {
 };
 
 TbCfg tb_cfg(int bytes, int K) {
-  // Sweeps at 512^3 and 1024^3 on MI355X, all bit-exact: FP64 runs best
-  // with few rows per wave (occupancy: 5 rows x 16 waves at K=6, 5 x 8 at
-  // K=8), FP32 with fewer redundant halo rows (9 rows x 8 waves at K>=6);
-  // K<=4: 9 rows x 4 waves.
+  // Sweeps at 512^3 and 1024^3 on MI355X, all bit-exact: 9 rows x 8 waves
+  // at K >= 6 (72 rows per block, 64 output rows at K=8) for FP64 and
+  // FP32; 9 rows x 4 waves at K <= 4.
   TbCfg c{9, 4, 0};
-  if (K >= 6) {
-    if (bytes == 8)
-      c = K >= 8 ? TbCfg{5, 8, 0} : TbCfg{5, 16, 0};
-    else
-      c = TbCfg{9, 8, 0};
-  }
+  if (K >= 6) c = TbCfg{9, 8, 0};
   char name[48];
   snprintf(name, sizeof(name), "FDB_TB_CFG_%d_%d", bytes, K);
   if (const char* s = getenv(name)) {
@@ -1179,11 +1175,10 @@ hipError_t tb_dispatch(const Grid3& g, const TS* in, TS* out, int K, hipStream_t
   FDB_TB(9, 2, 4)
   FDB_TB(9, 3, 4)
   FDB_TB(9, 4, 4)
-  FDB_TB(5, 6, 8)
   FDB_TB(9, 6, 8)
-  FDB_TB(5, 6, 16)
-  FDB_TB(5, 8, 8)
+  FDB_TB(9, 7, 8)
   FDB_TB(9, 8, 8)
+  FDB_TB(5, 8, 8)
 #undef FDB_TB
 #undef FDB_TB_DIR
   if (!launched) return hipErrorNotSupported;
@@ -1284,7 +1279,7 @@ hipError_t launch_advect_pass(int dtype, const Grid3& g, void* const bufs[2], in
     void* dst = bufs[1 - *cur];
     int done = 0;
     // Temporally blocked pass with the deepest supported K <= left.
-    for (int K : {8, 6, 4, 3, 2}) {
+    for (int K : {8, 7, 6, 4, 3, 2}) {
       if (K > left) continue;
       switch (dtype) {
         case FDB_F64: err = tb_dispatch<double>(g, (const double*)src, (double*)dst, K, s); break;
