@@ -457,8 +457,8 @@ int fdb_adjoint_step_dev(int ndims, const uint64_t* num_cells, const double* vel
   fdb::Grid3 g;
   if (!fdb::make_grid(ndims, num_cells, velocity, lengths, delta_time, &g))
     return fail(FDB_ERR_INVALID, "adjoint_step_dev: invalid geometry");
-  if (lam_dtype != FDB_F64 && lam_dtype != FDB_F32)
-    return fail(FDB_ERR_INVALID, "adjoint_step_dev: lam dtype must be F64 or F32");
+  if (lam_dtype != FDB_F64 && lam_dtype != FDB_F32 && lam_dtype != FDB_BF16)
+    return fail(FDB_ERR_INVALID, "adjoint_step_dev: lam dtype must be F64, F32 or BF16");
   if (f_t && !valid_dtype(f_dtype)) return fail(FDB_ERR_INVALID, "adjoint_step_dev: f dtype");
   if (!lam_next || !lam || lam_next == lam)
     return fail(FDB_ERR_INVALID, "adjoint_step_dev: need two distinct buffers");
@@ -490,7 +490,8 @@ int fdb_sum_dev(int dtype, const void* x, uint64_t n, double* out, void* stream)
 
 int fdb_misfit_dev(int f_dtype, const void* f, const void* target, uint64_t n, int lam_dtype,
                    void* lam, double* loss, void* stream) {
-  if (!valid_dtype(f_dtype) || (lam_dtype != FDB_F64 && lam_dtype != FDB_F32) || !f || !target ||
+  if (!valid_dtype(f_dtype) ||
+      (lam_dtype != FDB_F64 && lam_dtype != FDB_F32 && lam_dtype != FDB_BF16) || !f || !target ||
       !lam || !loss)
     return fail(FDB_ERR_INVALID, "misfit_dev: bad argument");
   double* part = nullptr;

@@ -1000,6 +1000,10 @@ RowsCfg rows_cfg(int bytes, bool adj, int fbytes = 0) {
         c.R = 4;
         c.WY = 4;
       }
+      if (bytes == 2) {  // bf16 lam: 8 cells per lane, as the bf16 forward
+        c.R = 2;
+        c.WY = 4;
+      }
     } else if (bytes == 4) {
       c.R = 4;
     } else if (bytes == 2) {
@@ -1045,8 +1049,9 @@ hipError_t rows_dispatch(const Grid3& g, const TS* in, TS* out, const int dirs[3
   a.act2 = g.act[2];
   Coef<M> cf;
   for (int j = 0; j < 3; ++j) cf.c[j] = (M)g.c[j];
-  // Widest access that keeps every row start aligned (f_t included).
-  int vec = max_vec<TS>();
+  // Widest access that keeps every row start aligned (f_t included, and
+  // no wider than 16 bytes of f_t).
+  int vec = std::min(max_vec<TS>(), max_vec<TF>());
   uintptr_t align_bits = (uintptr_t)in | (uintptr_t)out;
   while (vec > 1 && ((a.n2 % vec) != 0 || (align_bits % (vec * sizeof(TS))) != 0 ||
                      ((uintptr_t)ft % (vec * sizeof(TF))) != 0))
@@ -1193,7 +1198,8 @@ hipError_t step_typed(const Grid3& g, const void* in, void* out, hipStream_t s,
 }
 
 // The transpose step: directions -u; with f_t the velocity-gradient
-// products too (lam in FP64 or FP32, f_t in any storage type).
+// products too (lam in FP64, FP32 or bf16 storage with FP32 arithmetic;
+// f_t in any storage type).
 hipError_t adjoint_dispatch(int lam_dtype, int f_dtype, const Grid3& g, const void* lam_next,
                             void* lam, const void* f_t, double* partials, int64_t* nb,
                             hipStream_t s, bool dry) {
@@ -1208,6 +1214,9 @@ hipError_t adjoint_dispatch(int lam_dtype, int f_dtype, const Grid3& g, const vo
   if (lam_dtype == FDB_F32 && f_dtype == FDB_F64) FDB_ADJ(float, double);
   if (lam_dtype == FDB_F32 && f_dtype == FDB_F32) FDB_ADJ(float, float);
   if (lam_dtype == FDB_F32 && f_dtype == FDB_BF16) FDB_ADJ(float, uint16_t);
+  if (lam_dtype == FDB_BF16 && f_dtype == FDB_F64) FDB_ADJ(uint16_t, double);
+  if (lam_dtype == FDB_BF16 && f_dtype == FDB_F32) FDB_ADJ(uint16_t, float);
+  if (lam_dtype == FDB_BF16 && f_dtype == FDB_BF16) FDB_ADJ(uint16_t, uint16_t);
 #undef FDB_ADJ
   return hipErrorInvalidValue;
 }

@@ -296,8 +296,8 @@ int fdb_tape_create(int ndims, const uint64_t* num_cells, const double* velocity
     return tfail(FDB_ERR_INVALID, "tape_create: invalid geometry");
   if (nsteps < 1) return tfail(FDB_ERR_INVALID, "tape_create: nsteps must be >= 1");
   if (fdb::dtype_size(f_dtype) == 0) return tfail(FDB_ERR_INVALID, "tape_create: f dtype");
-  if (lam_dtype != FDB_F64 && lam_dtype != FDB_F32)
-    return tfail(FDB_ERR_INVALID, "tape_create: lam dtype must be F64 or F32");
+  if (lam_dtype != FDB_F64 && lam_dtype != FDB_F32 && lam_dtype != FDB_BF16)
+    return tfail(FDB_ERR_INVALID, "tape_create: lam dtype must be F64, F32 or BF16");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
     return tfail(FDB_ERR_NODEV, "no HIP device available");

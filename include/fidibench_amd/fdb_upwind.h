@@ -138,7 +138,8 @@ int fdb_advect_dev(int ndims, const uint64_t* num_cells, const double* velocity,
  * If f_t is non-null, also accumulates into vgrad_acc (device, 6 doubles:
  * three (sum, compensation) pairs) the velocity-gradient terms
  *   sum_i lam_next[i] * (f_t[up_j(i)] - f_t[i]) * (-(dt*u_j/dx_j)).
- * lam dtype is FDB_F64 or FDB_F32; f_t has dtype f_dtype. */
+ * lam dtype is FDB_F64, FDB_F32 or FDB_BF16 (FP32 arithmetic, each step
+ * rounded to nearest even); f_t has dtype f_dtype. */
 int fdb_adjoint_step_dev(int ndims, const uint64_t* num_cells, const double* velocity,
                          const double* lengths, double delta_time, int lam_dtype,
                          const void* lam_next, void* lam, int f_dtype, const void* f_t,
@@ -159,8 +160,9 @@ int fdb_misfit_dev(int f_dtype, const void* f, const void* target, uint64_t n, i
  * ndims doubles), recomputing the states it did not keep.  Checkpoints
  * follow the optimal binomial schedule (Griewank's revolve) for the
  * number of field copies that fit in max_bytes of device memory.
- * f_dtype is the forward storage type, lam_dtype (FDB_F64 or FDB_F32)
- * the adjoint's.  All device buffers are caller-owned; both calls block. */
+ * f_dtype is the forward storage type, lam_dtype (FDB_F64, FDB_F32 or
+ * FDB_BF16) the adjoint's.  All device buffers are caller-owned; both
+ * calls block. */
 typedef struct fdb_tape fdb_tape_t;
 int fdb_tape_create(int ndims, const uint64_t* num_cells, const double* velocity,
                     const double* lengths, double delta_time, int nsteps, int f_dtype,

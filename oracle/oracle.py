@@ -53,6 +53,10 @@ def lib() -> C.CDLL:
             "orc_sr_bits": (C.c_uint32, [C.c_uint64, C.c_uint32]),
             "orc_sr_step_key": (C.c_uint32, [C.c_uint64, C.c_uint64]),
             "orc_adjoint_f64": (C.c_int, [C.c_int, _u64p, _dp, _dp, C.c_double, C.c_int, _dp]),
+            "orc_adjoint_step_lowp": (C.c_int, [C.c_int, _u64p, _dp, _dp, C.c_double, C.c_int,
+                                                C.c_void_p, C.c_void_p]),
+            "orc_vgrad_terms": (C.c_int, [C.c_int, _u64p, _dp, _dp, C.c_double, C.c_int,
+                                          C.c_void_p, C.c_int, C.c_void_p, _dp]),
             "orc_misfit_grad_f64": (C.c_int, [C.c_int, _u64p, _dp, _dp, C.c_double, C.c_int,
                                               _dp, _dp, _dp, _dp, _dp]),
             "orc_tlm_f64": (C.c_int, [C.c_int, _u64p, _dp, _dp, C.c_double, C.c_int,
@@ -157,6 +161,36 @@ def adjoint(lam: np.ndarray, v, lengths, dt: float, nsteps: int) -> np.ndarray:
     n = lam.shape
     out = np.ascontiguousarray(lam, dtype=np.float64).copy()
     rc = lib().orc_adjoint_f64(len(n), _u64(n), _d(v), _d(lengths), dt, nsteps, _ptr(out, C.c_double))
+    if rc:
+        raise RuntimeError(rc)
+    return out
+
+
+_KIND = {np.dtype(np.float64): 0, np.dtype(np.float32): 1, np.dtype(np.uint16): 2}
+
+
+def adjoint_step_lowp(lam_next: np.ndarray, v, lengths, dt: float) -> np.ndarray:
+    """One adjoint step in float arithmetic on float32 or bf16 (uint16)
+    storage, the operation order of the FP64 adjoint."""
+    a = np.ascontiguousarray(lam_next)
+    kind = _KIND[a.dtype]
+    assert kind in (1, 2)
+    out = np.empty_like(a)
+    rc = lib().orc_adjoint_step_lowp(a.ndim, _u64(a.shape), _d(v), _d(lengths), dt, kind,
+                                     a.ctypes.data, out.ctypes.data)
+    if rc:
+        raise RuntimeError(rc)
+    return out
+
+
+def vgrad_terms(lam_next: np.ndarray, f_t: np.ndarray, v, lengths, dt: float) -> np.ndarray:
+    """D_j = sum_i f_t[i] (lam_next[i - u_j e_j] - lam_next[i]) per axis
+    (float64, bf16 as uint16 bit patterns), long-double sums."""
+    a, f = np.ascontiguousarray(lam_next), np.ascontiguousarray(f_t)
+    out = np.zeros(a.ndim)
+    rc = lib().orc_vgrad_terms(a.ndim, _u64(a.shape), _d(v), _d(lengths), dt, _KIND[a.dtype],
+                               a.ctypes.data, _KIND[f.dtype], f.ctypes.data,
+                               out.ctypes.data_as(C.POINTER(C.c_double)))
     if rc:
         raise RuntimeError(rc)
     return out

@@ -289,6 +289,66 @@ void orc_adjoint_step_f64(int nd, const uint64_t* n, const double* v, const doub
   }
 }
 
+/* Storage kinds of the mixed-precision helpers: 0 FP64, 1 FP32, 2 bf16. */
+static double orc_widen(int kind, const void* p, uint64_t i) {
+  if (kind == 0) return ((const double*)p)[i];
+  if (kind == 1) return (double)((const float*)p)[i];
+  return (double)orc_bf16_to_f32(((const uint16_t*)p)[i]);
+}
+
+/* One adjoint step with float arithmetic (the FP64 coefficients rounded
+ * once to float) on float (kind 1) or bfloat16 (kind 2: widened, result
+ * rounded to nearest even) storage, in the operation order of
+ * orc_adjoint_step_f64 -- the reduced-precision variants of the forward
+ * (orc_advect_f32/bf16) applied to the transpose. */
+int orc_adjoint_step_lowp(int nd, const uint64_t* n, const double* v, const double* len,
+                          double dt, int kind, const void* lam_next, void* lam) {
+  orc_geom g;
+  if ((kind != 1 && kind != 2) || orc_geom_init(&g, nd, n, v, len, dt)) return -1;
+  float cf[ORC_MAXD];
+  for (int j = 0; j < nd; ++j) cf[j] = (float)g.coeff[j];
+  const int64_t ntot = (int64_t)g.ntot;
+  float* w = (float*)malloc(g.ntot * sizeof(float));
+  if (!w) return -2;
+  for (int64_t i = 0; i < ntot; ++i) w[i] = (float)orc_widen(kind, lam_next, (uint64_t)i);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < ntot; ++i) {
+    float li = w[i];
+    for (int j = 0; j < nd; ++j) {
+      uint64_t dn = orc_shift(&g, (uint64_t)i, j, -g.up[j]);
+      li = li - cf[j] * (w[dn] - w[i]);
+    }
+    if (kind == 1)
+      ((float*)lam)[i] = li;
+    else
+      ((uint16_t*)lam)[i] = orc_f32_to_bf16(li);
+  }
+  free(w);
+  return 0;
+}
+
+/* The velocity-gradient dot products of one adjoint step,
+ *   D_j = sum_i f_t[i] * (lam_next[i - u_j e_j] - lam_next[i]),
+ * with each value widened to double, each difference and product in
+ * double, and the sum a long-double left fold (dL/dv_j gets
+ * -(dt*u_j/dx_j) * D_j per step).  Kinds as orc_widen. */
+int orc_vgrad_terms(int nd, const uint64_t* n, const double* v, const double* len, double dt,
+                    int lam_kind, const void* lam_next, int f_kind, const void* f_t,
+                    double* out) {
+  orc_geom g;
+  if (orc_geom_init(&g, nd, n, v, len, dt)) return -1;
+  for (int j = 0; j < nd; ++j) {
+    long double s = 0.0L;
+    for (uint64_t i = 0; i < g.ntot; ++i) {
+      uint64_t dn = orc_shift(&g, i, j, -g.up[j]);
+      const double d = orc_widen(lam_kind, lam_next, dn) - orc_widen(lam_kind, lam_next, i);
+      s += (long double)(orc_widen(f_kind, f_t, i) * d);
+    }
+    out[j] = (double)s;
+  }
+  return 0;
+}
+
 int orc_adjoint_f64(int nd, const uint64_t* n, const double* v, const double* len,
                     double dt, int nsteps, double* lam) {
   orc_geom g;

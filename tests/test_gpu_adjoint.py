@@ -46,6 +46,99 @@ def test_adjoint_step_bit_exact_vs_oracle(oracle):
         assert np.array_equal(bits(b.cpu().numpy()), bits(ref)), shape
 
 
+def _dev(arr):
+    """numpy array (float64/float32, or bf16 as uint16) -> CUDA tensor, fdb dtype."""
+    from fidibench_amd import _lib as L
+    if arr.dtype == np.uint16:
+        return torch.from_numpy(arr.view(np.int16).copy()).cuda().view(torch.bfloat16), L.BF16
+    code = L.F64 if arr.dtype == np.float64 else L.F32
+    return torch.from_numpy(arr.copy()).cuda(), code
+
+
+def _host(t):
+    if t.dtype == torch.bfloat16:
+        return t.view(torch.int16).cpu().numpy().view(np.uint16)
+    return t.cpu().numpy()
+
+
+def _as(oracle, x, kind):
+    if kind == "f64":
+        return x
+    if kind == "f32":
+        return x.astype(np.float32)
+    return oracle.f32_to_bf16(x.astype(np.float32)).reshape(x.shape)
+
+
+@pytest.mark.parametrize("lam_kind", ["f32", "bf16"])
+def test_lowp_adjoint_step_bit_exact_vs_oracle(oracle, lam_kind):
+    """FP32 and bf16-storage adjoint steps against the oracle's
+    float-arithmetic restatement, every sign branch, partial tiles."""
+    from fidibench_amd import _lib as L
+    for shape, v in [((10, 12, 70), (0.45, -0.3, 0.2)), ((7, 9, 130), (-0.2, 0.6, -0.5)),
+                     ((5, 17, 64), (0.3, 0.1, -0.7)), ((4, 6, 40), (-0.3, -0.2, 0.5))]:
+        x = _as(oracle, oracle.uniform(shape, -1, 1, seed=3), lam_kind)
+        ref = oracle.adjoint_step_lowp(x, v, (1, 1, 1), 0.013)
+        a, code = _dev(x)
+        b = torch.empty_like(a)
+        L.check(L.lib().fdb_adjoint_step_dev(3, L.u64s(shape), L.dbls(v), L.dbls((1, 1, 1)), 0.013,
+                                             code, a.data_ptr(), b.data_ptr(), code, None, None,
+                                             torch.cuda.current_stream().cuda_stream))
+        torch.cuda.synchronize()
+        assert np.array_equal(bits(_host(b)), bits(ref)), (shape, lam_kind)
+
+
+@pytest.mark.parametrize("lam_kind,f_kind", [("f64", "f64"), ("f64", "bf16"), ("f32", "f32"),
+                                             ("f32", "bf16"), ("bf16", "bf16"), ("bf16", "f64")])
+def test_adjoint_velocity_terms_vs_oracle(oracle, lam_kind, f_kind):
+    """The fused velocity-gradient dot products of one adjoint step
+    (compensated FP64 on the device) against long-double sums."""
+    from fidibench_amd import _lib as L
+    shape, v, dt = (9, 13, 96), (0.35, -0.45, 0.25), 0.011
+    lam = _as(oracle, oracle.uniform(shape, -1, 1, seed=21), lam_kind)
+    f = _as(oracle, oracle.uniform(shape, 0, 1, seed=22), f_kind)
+    ref = oracle.vgrad_terms(lam, f, v, (1, 1, 1), dt)
+    a, lcode = _dev(lam)
+    ft, fcode = _dev(f)
+    b = torch.empty_like(a)
+    acc = torch.zeros(6, dtype=torch.float64, device="cuda")
+    L.check(L.lib().fdb_adjoint_step_dev(3, L.u64s(shape), L.dbls(v), L.dbls((1, 1, 1)), dt, lcode,
+                                         a.data_ptr(), b.data_ptr(), fcode, ft.data_ptr(),
+                                         acc.data_ptr(), torch.cuda.current_stream().cuda_stream))
+    torch.cuda.synchronize()
+    got = acc.cpu().numpy()
+    d = got[0::2] + got[1::2]
+    assert np.allclose(d, ref, rtol=1e-11, atol=0), (d, ref)
+
+
+def test_bf16_adjoint_gradient_close_to_fp64():
+    """bf16-storage forward and bf16 adjoint through the tape against FP64:
+    bf16 keeps ~3 significant digits per stored value."""
+    from fidibench_amd import _lib as L
+    from fidibench_amd.autograd import Tape
+    shape, v, dt, T = (16, 16, 64), (0.45, -0.3, 0.2), 0.004, 10
+    g = torch.Generator().manual_seed(12)
+    f64 = torch.rand(shape, generator=g, dtype=torch.float64)
+    lam64 = torch.rand(shape, generator=g, dtype=torch.float64)
+    out = {}
+    for name, fd, ld, tdt in (("f64", L.F64, L.F64, torch.float64),
+                              ("bf16", L.BF16, L.BF16, torch.bfloat16)):
+        f0 = f64.to(tdt).cuda()
+        tape = Tape(shape, v, (1, 1, 1), dt, T, fd, ld, 1 << 30)
+        fT = torch.empty_like(f0)
+        tape.forward(f0, fT)
+        lamT = lam64.to(tdt).cuda()
+        lam0 = torch.empty_like(lamT)
+        gv = tape.backward(lamT, lam0)
+        out[name] = (lam0.double().cpu().numpy(), np.array(gv))
+        tape.close()
+    (l64, g64), (l16, g16) = out["f64"], out["bf16"]
+    # dL/dv sums products of neighbour differences of lam, which bf16
+    # storage resolves to ~3 digits, so the bound is on the largest
+    # component (measured ~0.5 % for dL/df0, ~2 % for dL/dv here).
+    assert np.abs(l16 - l64).max() <= 3e-2 * np.abs(l64).max()
+    assert np.abs(g16 - g64).max() <= 3e-2 * np.abs(g64).max(), (g16, g64)
+
+
 def test_dot_product_identity_on_gpu():
     from fidibench_amd import _lib as L
     shape, v, dt = (16, 20, 130), (0.3, -0.45, 0.25), 0.002
@@ -73,7 +166,7 @@ def _tape_grad(f0, target, v, dt, T, max_bytes=None, f_dtype=torch.float64):
     fT = advect(f0t, vt, dt, T, max_bytes=max_bytes)
     loss = misfit(fT, torch.from_numpy(target).cuda().to(f_dtype))
     loss.backward()
-    return float(loss), f0t.grad.double().cpu().numpy(), vt.grad.numpy().copy(), fT
+    return float(loss.detach()), f0t.grad.double().cpu().numpy(), vt.grad.numpy().copy(), fT
 
 
 def test_gradient_matches_oracle(oracle):

@@ -255,6 +255,45 @@ def test_bf16_sr_advect_oracle(oracle):
     assert np.array_equal(a, h)
 
 
+def test_lowp_adjoint_step_oracle(oracle):
+    """The float/bf16 adjoint step: on dyadic data where every intermediate
+    is exact in float and the result exact in bf16, it equals the FP64
+    adjoint step (pinned by the dot-product identity) bit for bit, for every
+    sign combination; a dropped term, a wrong gather direction or a wrong
+    coefficient breaks the equality."""
+    import itertools
+    rng = np.random.default_rng(9)
+    shape = (6, 8, 16)
+    lam = rng.integers(-8, 9, size=shape).astype(np.float64) / 8.0
+    for signs in itertools.product((1, -1), repeat=3):
+        v = tuple(s * m for s, m in zip(signs, (16.0 / 6, 8.0 / 8, 8.0 / 16)))
+        dt = 1.0 / 64  # c = 1/4, 1/8, 1/8
+        ref = oracle.adjoint(lam, v, (1, 1, 1), dt, 1)
+        assert not np.array_equal(ref, lam)
+        a32 = oracle.adjoint_step_lowp(lam.astype(np.float32), v, (1, 1, 1), dt)
+        assert np.array_equal(a32.astype(np.float64), ref), signs
+        b16 = oracle.f32_to_bf16(lam.astype(np.float32)).reshape(shape)
+        a16 = oracle.adjoint_step_lowp(b16, v, (1, 1, 1), dt)
+        assert np.array_equal(oracle.bf16_to_f32(a16).astype(np.float64).reshape(shape), ref)
+
+
+def test_vgrad_terms_oracle(oracle):
+    """One-step misfit gradient: dL/dv_j = -(dt u_j/dx_j) D_j(lam_1, f_0)
+    with lam_1 = f_1 - target, against the FP64 misfit gradient (pinned by
+    the TLM and finite differences)."""
+    rng = np.random.default_rng(4)
+    shape, v, dt = (7, 9, 20), (0.4, -0.3, 0.25), 0.02
+    f0 = rng.random(shape)
+    target = rng.random(shape)
+    _, _, gv = oracle.misfit_grad(f0, target, v, (1, 1, 1), dt, 1)
+    f1 = oracle.advect(f0, v, (1, 1, 1), dt, 1)
+    d = oracle.vgrad_terms(f1 - target, f0, v, (1, 1, 1), dt)
+    for j in range(3):
+        u = 1.0 if v[j] < 0 else -1.0
+        dc = dt * u / (1.0 / shape[j])
+        assert abs(-dc * d[j] - gv[j]) <= 1e-12 * abs(gv[j]), j
+
+
 def test_f32_error_vs_f64_is_small(oracle):
     n, v = (24, 24, 24), (0.7, -0.4, 0.2)
     from fidibench_amd import configs as K
