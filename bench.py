@@ -11,13 +11,18 @@ advect steps between two device synchronisations, and kernel time is
 measured with HIP events on the stream the kernels run on
 (fdb_upwind_advect_timed).
 
-One JSON line: value = FP64 cell-updates/s (bit-exact with the reference,
+One JSON line: value = FP64 cell-updates/s at the best time-blocking depth
+k (steps fused per launch; bit-exact with the reference for every k,
 checked against its 512^3 x 100 checksum), plus variants:
-  f32, bf16        the same workload with 4- and 2-byte storage
-  mixed_gauss1024  1024^3 x 200 Gaussian bubble in FP64/FP32/BF16: rates,
-                   max-abs / RMS / max-rel error vs FP64, mass drift
+  {f64,f32,bf16}_k{k}  the workload per storage type and depth
+  mixed_gauss1024  1024^3 x 200 Gaussian bubble in FP64/FP32/BF16 (round to
+                   nearest and stochastic): rates, max-abs / RMS / max-rel
+                   error vs FP64, mass drift
   fwd_adj_*        forward + adjoint (dL/df0, dL/dv of a misfit) at 512^3
-                   over the same steps, time ratio to the forward alone
+                   over the same steps, time ratio to the one-step forward
+  adjoint_step     one adjoint step per storage type (24 / 12 / 6 B a cell)
+  adjoint_large    1024x1024x768 x 400 checkpointed gradient, BF16-SR/FP32
+                   against FP64
 and the roofline of the dominant kernel and the CPU baseline (the
 reference's own upwind.cxx, compiled by oracle/Makefile, timed on this
 host's cores on a bounded sample of the same workload).
@@ -171,8 +176,8 @@ def forward_adjoint(w, steps, f_dtype, fwd_ms):
     from fidibench_amd import _lib as L
     from fidibench_amd.autograd import Tape, default_budget
     from fidibench_amd import configs as K
-    tdt = {"f64": torch.float64, "f32": torch.float32}[f_dtype]
-    fd = {"f64": L.F64, "f32": L.F32}[f_dtype]
+    tdt = {"f64": torch.float64, "f32": torch.float32, "bf16": torch.bfloat16}[f_dtype]
+    fd = {"f64": L.F64, "f32": L.F32, "bf16": L.BF16}[f_dtype]
     f0 = torch.empty(w.num_cells, dtype=tdt, device="cuda")
     L.check(L.lib().fdb_init_field_dev(3, L.u64s(w.num_cells), L.dbls(w.lengths), fd, w.init_kind,
                                        L.dbls(w.init_params), len(w.init_params), w.init_seed,
@@ -198,6 +203,45 @@ def forward_adjoint(w, steps, f_dtype, fwd_ms):
             "cell_steps_per_s": w.ntot * steps / (tot * 1e-3), "snapshots": st["snapshots"],
             "forward_steps": st["forward_steps"], "adjoint_steps": st["adjoint_steps"],
             "grad_v": gv}
+
+
+def adjoint_step_rates(w, reps=10):
+    """One adjoint step with the velocity-gradient products (reads lam_{t+1}
+    and f_t, writes lam_t) per storage type, HIP-event timed over `reps`
+    steps: FP64 moves 24 B per cell, FP32 12 B, bf16 6 B."""
+    import torch
+    from fidibench_amd import _lib as L
+    out = {}
+    s = torch.cuda.current_stream().cuda_stream
+    for name, tdt, code in (("f64", torch.float64, L.F64), ("f32", torch.float32, L.F32),
+                            ("bf16", torch.bfloat16, L.BF16)):
+        lam_next = torch.rand(w.num_cells, dtype=torch.float32, device="cuda").to(tdt)
+        lam = torch.empty_like(lam_next)
+        ft = torch.rand(w.num_cells, dtype=torch.float32, device="cuda").to(tdt)
+        acc = torch.zeros(6, dtype=torch.float64, device="cuda")
+
+        def step():
+            L.check(L.lib().fdb_adjoint_step_dev(3, L.u64s(w.num_cells), L.dbls(w.velocity),
+                                                 L.dbls(w.lengths), w.dt, code,
+                                                 lam_next.data_ptr(), lam.data_ptr(), code,
+                                                 ft.data_ptr(), acc.data_ptr(), s))
+        step()
+        sync()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            step()
+        e1.record()
+        sync()
+        ms = e0.elapsed_time(e1) / reps
+        nbytes = w.ntot * 3 * lam.element_size()
+        out[name] = {"ms_per_step": ms, "cell_steps_per_s": w.ntot / (ms * 1e-3),
+                     "bytes_per_cell": 3 * lam.element_size(),
+                     "hbm_frac_compulsory": nbytes / (ms * 1e-3) / HBM_PEAK}
+        del lam_next, lam, ft
+    for name in ("f32", "bf16"):
+        out[name]["speedup_vs_f64"] = out["f64"]["ms_per_step"] / out[name]["ms_per_step"]
+    return out
 
 
 def large_adjoint(shape=(1024, 1024, 768), steps=400):
@@ -298,6 +342,9 @@ def main():
         res["fwd_adj_f64"] = forward_adjoint(w, a.steps, "f64", fwd_ms)
         f32_ms = res.get("f32_k1", {}).get("ms")
         res["fwd_adj_f32"] = forward_adjoint(w, a.steps, "f32", f32_ms)
+        bf16_ms = res.get("bf16_k1", {}).get("ms")
+        res["fwd_adj_bf16"] = forward_adjoint(w, a.steps, "bf16", bf16_ms)
+        res["adjoint_step"] = adjoint_step_rates(w)
     if "adjlarge" in only:
         res["adjoint_large"] = large_adjoint()
 

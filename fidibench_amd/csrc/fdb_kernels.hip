@@ -1000,9 +1000,9 @@ RowsCfg rows_cfg(int bytes, bool adj, int fbytes = 0) {
         c.R = 4;
         c.WY = 4;
       }
-      if (bytes == 2) {  // bf16 lam: 8 cells per lane, as the bf16 forward
-        c.R = 2;
-        c.WY = 4;
+      if (bytes == 2) {  // bf16 lam: 8 cells per lane, 1 row x 8 waves (512^3 sweep)
+        c.R = 1;
+        c.WY = 8;
       }
     } else if (bytes == 4) {
       c.R = 4;
