@@ -66,6 +66,18 @@ SIGNATURES = {
     "fdb_host_std": (C.c_double, [_dp, C.c_uint64]),
     "fdb_host_save_vtk": (_i, [_i, _u64p, _dp, _dp, C.c_char_p]),
     "fdb_ckpt_plan": (_i, [_i, _i, C.POINTER(_i), C.POINTER(_i), _i, C.POINTER(C.c_int64)]),
+    # fdb_filter.h
+    "fdb_filter_create": (_i, [_i, _u64p, _dp, _dp, _i, C.POINTER(_i), _dp, C.POINTER(_vp)]),
+    "fdb_filter_destroy": (_i, [_vp]),
+    "fdb_filter_set_in_data": (_i, [_vp, _dp]),
+    "fdb_filter_set_in_sin": (_i, [_vp]),
+    "fdb_filter_apply": (_i, [_vp]),
+    "fdb_filter_copy_out_to_in": (_i, [_vp]),
+    "fdb_filter_iterate_timed": (_i, [_vp, _i, C.POINTER(C.c_float)]),
+    "fdb_filter_checksum": (_i, [_vp, _i, _dp]),
+    "fdb_filter_get_data": (_i, [_vp, _i, _dp]),
+    "fdb_filter_save_vtk": (_i, [_vp, C.c_char_p]),
+    "fdb_host_write_vtk": (_i, [_i, _u64p, _dp, _dp, _dp, C.c_char_p, C.c_char_p]),
 }
 
 _lib = None

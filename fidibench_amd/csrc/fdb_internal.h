@@ -64,6 +64,34 @@ hipError_t launch_fused_pass(int dtype, const Grid3& g, const void* src, void* d
 // The storage layout of a dtype (FDB_BF16_SR is stored as FDB_BF16).
 inline int storage_dtype(int dtype) { return dtype == 3 ? 2 : dtype; }
 
+// ---- generic periodic stencil (cxx/Filter.cpp), kernels in fdb_filter.hip
+
+constexpr int kFilterMaxTerms = 32;   // stencil terms
+constexpr int kFilterMaxOffset = 8;   // |offset| per axis
+
+// Geometry of a stencil filter.  Axes are the reference's: axis 0 is the
+// contiguous one (Filter's MultiArrayIter is column-major); missing axes
+// have extent 1 and offset 0.  The terms are in std::map order of their
+// offset vectors.  neg[a][k-1] is where index -k lands along axis a under
+// the reference's wrap, (size_t)(-k) % n[a] (cxx/Filter.cpp:231-233).
+struct FilterGeom {
+  int64_t n[3];
+  int64_t ntot;
+  int nt;
+  int off[kFilterMaxTerms][3];
+  double w[kFilterMaxTerms];
+  int64_t neg[3][kFilterMaxOffset];
+};
+
+// out[i] = sum over terms, in order, of w * in[wrapped neighbour].
+hipError_t launch_filter(const FilterGeom& g, const double* in, double* out, hipStream_t s);
+
+// in[i] = prod_a sin(2 pi x_a), x_a = xmin_a + (i_a + 0.5) * (xmax_a - xmin_a) / n_a
+// (the input function of laplacian/cxx/laplacian.cxx:22-28), with the
+// device's sin (not bit-identical to the host's libm).
+hipError_t launch_filter_init_sin(const FilterGeom& g, int ndims, const double* xmins,
+                                  const double* xmaxs, double* in, hipStream_t s);
+
 // lam = A^T lam_next; if f_t != nullptr also adds the three velocity
 // gradient dot products into acc (device, 3 x {sum, comp}).
 hipError_t launch_adjoint_step(int lam_dtype, int f_dtype, const Grid3& g, const void* lam_next,

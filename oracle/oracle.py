@@ -57,6 +57,8 @@ def lib() -> C.CDLL:
                                                 C.c_void_p, C.c_void_p]),
             "orc_vgrad_terms": (C.c_int, [C.c_int, _u64p, _dp, _dp, C.c_double, C.c_int,
                                           C.c_void_p, C.c_int, C.c_void_p, _dp]),
+            "orc_filter_apply": (C.c_int, [C.c_int, _u64p, C.c_int, C.POINTER(C.c_int), _dp,
+                                           _dp, _dp]),
             "orc_misfit_grad_f64": (C.c_int, [C.c_int, _u64p, _dp, _dp, C.c_double, C.c_int,
                                               _dp, _dp, _dp, _dp, _dp]),
             "orc_tlm_f64": (C.c_int, [C.c_int, _u64p, _dp, _dp, C.c_double, C.c_int,
@@ -191,6 +193,23 @@ def vgrad_terms(lam_next: np.ndarray, f_t: np.ndarray, v, lengths, dt: float) ->
     rc = lib().orc_vgrad_terms(a.ndim, _u64(a.shape), _d(v), _d(lengths), dt, _KIND[a.dtype],
                                a.ctypes.data, _KIND[f.dtype], f.ctypes.data,
                                out.ctypes.data_as(C.POINTER(C.c_double)))
+    if rc:
+        raise RuntimeError(rc)
+    return out
+
+
+def filter_apply(data: np.ndarray, dims, stencil) -> np.ndarray:
+    """One applyFilter() (cxx/Filter.cpp, one process) on a column-major
+    flat array `data` of prod(dims) cells; stencil = list of (offset
+    tuple, weight), in insertion order (std::map semantics applied)."""
+    d = np.ascontiguousarray(data, dtype=np.float64).ravel()
+    nd = len(dims)
+    offs = np.array([o for o, _ in stencil], dtype=np.int32).reshape(-1, nd)
+    w = np.array([x for _, x in stencil], dtype=np.float64)
+    out = np.empty_like(d)
+    rc = lib().orc_filter_apply(nd, _u64(dims), len(w),
+                                offs.ctypes.data_as(C.POINTER(C.c_int)), _ptr(w, C.c_double),
+                                _ptr(d, C.c_double), _ptr(out, C.c_double))
     if rc:
         raise RuntimeError(rc)
     return out

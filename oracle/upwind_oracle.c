@@ -27,11 +27,17 @@
  * mathematics (dot-product test, exact dyadic arithmetic, closed forms,
  * finite differences):
  *   orc_advect_f32 / orc_advect_bf16  reduced-precision storage, float math
+ *   orc_advect_bf16_sr                bf16 storage, stochastic rounding
  *   orc_adjoint_step_f64              exact transpose of one forward step
+ *   orc_adjoint_step_lowp             the same in float math, f32/bf16 storage
+ *   orc_vgrad_terms                   velocity-gradient dot products
  *   orc_misfit_grad_f64               loss, dL/df0 and dL/dv by adjoint
  *   orc_tlm_f64                       tangent-linear model (directional
  *                                     derivative of f_T)
  *   orc_init_*                        synthetic initial fields
+ *
+ * Also the generic stencil filter of cxx/Filter.cpp (one process),
+ * orc_filter_apply, used to check the GPU Filter.
  */
 #include <math.h>
 #include <stdint.h>
@@ -513,4 +519,71 @@ double orc_wtime(void) {
 #else
   return 0.0;
 #endif
+}
+
+/* ------------------------------------------------------------- filter --- */
+
+/* One applyFilter() of pletzer/fidibench cxx/Filter.cpp:192-263 with one
+ * process (every neighbour local).  Column-major cells (axis 0 fastest,
+ * Filter.cpp:50); the terms are taken in std::map order of their offset
+ * vectors (sorted here, lexicographically; of equal offsets the first is
+ * kept, as std::map::insert does); per cell out = 0, then out += w * in[nbr]
+ * term by term.  The neighbour index per axis is ((int)i + offset) % n with
+ * an unsigned (size_t) n, so a negative index wraps as its two's-complement
+ * unsigned value (Filter.cpp:231-233). */
+typedef struct {
+  int off[ORC_MAXD];
+  double w;
+  int order;
+} orc_term;
+
+static int orc_ndims_cmp;
+static int orc_term_cmp(const void* a, const void* b) {
+  const orc_term* x = (const orc_term*)a;
+  const orc_term* y = (const orc_term*)b;
+  for (int j = 0; j < orc_ndims_cmp; ++j)
+    if (x->off[j] != y->off[j]) return x->off[j] < y->off[j] ? -1 : 1;
+  return x->order - y->order;
+}
+
+int orc_filter_apply(int nd, const uint64_t* n, int nterms, const int* offsets,
+                     const double* weights, const double* in, double* out) {
+  if (nd < 1 || nd > ORC_MAXD || nterms < 1) return -1;
+  orc_term* t = (orc_term*)malloc(sizeof(orc_term) * (size_t)nterms);
+  if (!t) return -2;
+  for (int k = 0; k < nterms; ++k) {
+    for (int j = 0; j < nd; ++j) t[k].off[j] = offsets[(size_t)k * nd + j];
+    t[k].w = weights[k];
+    t[k].order = k;
+  }
+  orc_ndims_cmp = nd;
+  qsort(t, (size_t)nterms, sizeof(orc_term), orc_term_cmp);
+  int m = 0;  /* drop repeated offsets, keeping the first inserted */
+  for (int k = 0; k < nterms; ++k) {
+    int same = m > 0;
+    for (int j = 0; j < nd && same; ++j) same = t[k].off[j] == t[m - 1].off[j];
+    if (!same) t[m++] = t[k];
+  }
+  uint64_t ntot = 1, stride[ORC_MAXD];
+  for (int j = 0; j < nd; ++j) {
+    stride[j] = ntot;
+    ntot *= n[j];
+  }
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)ntot; ++i) {
+    double acc = 0;
+    for (int k = 0; k < m; ++k) {
+      uint64_t nb = 0;
+      for (int j = 0; j < nd; ++j) {
+        const uint64_t ij = ((uint64_t)i / stride[j]) % n[j];
+        const int64_t shifted = (int64_t)ij + t[k].off[j];
+        const uint64_t wrapped = (uint64_t)shifted % n[j];
+        nb += wrapped * stride[j];
+      }
+      acc += t[k].w * in[nb];
+    }
+    out[i] = acc;
+  }
+  free(t);
+  return 0;
 }

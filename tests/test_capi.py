@@ -14,7 +14,8 @@ import pytest
 
 from conftest import GOLDEN, ROOT, load_case
 
-HEADER = os.path.join(ROOT, "include", "fidibench_amd", "fdb_upwind.h")
+HEADERS = [os.path.join(ROOT, "include", "fidibench_amd", h)
+           for h in ("fdb_upwind.h", "fdb_filter.h")]
 
 
 def _lib():
@@ -23,9 +24,11 @@ def _lib():
 
 
 def declared_functions():
-    src = open(HEADER).read()
-    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(fdb_[a-z0-9_]+)\s*\(", src)))
+    names = set()
+    for h in HEADERS:
+        src = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names |= set(re.findall(r"\b(fdb_[a-z0-9_]+)\s*\(", src))
+    return sorted(names)
 
 
 def test_every_declared_symbol_is_exported():
