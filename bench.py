@@ -23,6 +23,8 @@ checked against its 512^3 x 100 checksum), plus variants:
   adjoint_step     one adjoint step per storage type (24 / 12 / 6 B a cell)
   adjoint_large    1024x1024x768 x 400 checkpointed gradient, BF16-SR/FP32
                    against FP64
+  laplacian        the stencil filter (cxx/Filter.cpp) on laplacian.cxx's
+                   2-D 8000^2 and a 3-D 512^3 Laplacian, 10 iterations
 and the roofline of the dominant kernel and the CPU baseline (the
 reference's own upwind.cxx, compiled by oracle/Makefile, timed on this
 host's cores on a bounded sample of the same workload).
@@ -43,7 +45,7 @@ sys.path.insert(0, ROOT)
 METRIC = "cell-updates/s & %HBM roofline at 512³: FP64 plain vs FP32/BF16 vs fwd+adjoint"
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 BYTES = {"f64": 8, "f32": 4, "bf16": 2, "bf16sr": 2}
-ALL = ("f64", "f32", "bf16", "mixed", "fwdadj", "adjlarge")
+ALL = ("f64", "f32", "bf16", "mixed", "fwdadj", "adjlarge", "laplacian")
 
 
 def parse():
@@ -244,6 +246,56 @@ def adjoint_step_rates(w, reps=10):
     return out
 
 
+def laplacian(iters=10, with_cpu=True):
+    """The stencil filter of cxx/Filter.cpp (one process) on laplacian.cxx's
+    workloads: the 5-point 2-D Laplacian at its default 8000^2 and the
+    7-point 3-D one at 512^3, `iters` x (applyFilter; copyOutToIn), input
+    prod sin(2 pi x) made on the GPU.  16 B per cell-update compulsory
+    (read input, write output); the oracle port (bit-identical result,
+    checked) is timed on the host for one iteration as the CPU figure."""
+    import time
+    import numpy as np
+    from fidibench_amd import Filter
+    out = {}
+    for name, nd, n in (("lap2d_8000", 2, 8000), ("lap3d_512", 3, 512)):
+        st = {tuple([0] * nd): -2.0 * nd}
+        for a in range(nd):
+            for s in (1, -1):
+                o = [0] * nd
+                o[a] = s
+                st[tuple(o)] = 1.0
+        dims = (n,) * nd
+        flt = Filter(dims, [0.0] * nd, [1.0] * nd, st)
+        flt.set_in_sin()
+        flt.iterate(2)  # warm-up
+        flt.set_in_sin()
+        ms = flt.iterate(iters)
+        ntot = n ** nd
+        cu = ntot * iters / (ms * 1e-3)
+        entry = {"cells": ntot, "iterations": iters, "ms": ms, "cell_updates_per_s": cu,
+                 "hbm_frac_compulsory": cu * 16 / HBM_PEAK,
+                 "checksum_output": flt.computeCheckSum("output")}
+        if name == "lap2d_8000" and with_cpu:
+            try:
+                from oracle import oracle as O
+                f0 = flt.get_data("input").ravel(order="F").copy()
+                flt.applyFilter()
+                gpu1 = flt.get_data("output").ravel(order="F")
+                t0 = time.perf_counter()
+                cpu1 = O.filter_apply(f0, dims, list(st.items()))
+                t = time.perf_counter() - t0
+                entry["cpu_port"] = {"cell_updates_per_s": ntot / t,
+                                     "threads": O.lib().orc_num_threads(),
+                                     "sample": "one iteration of the same grid"}
+                entry["bit_exact_vs_oracle"] = bool(np.array_equal(gpu1.view(np.uint64),
+                                                                   cpu1.view(np.uint64)))
+            except Exception as e:  # the CPU figure is reported, never required
+                entry["cpu_port"] = {"error": str(e)}
+        out[name] = entry
+        flt.close()
+    return out
+
+
 def large_adjoint(shape=(1024, 1024, 768), steps=400):
     """The checkpointed gradient at 1024x1024x768 x 400 steps: BF16-storage
     forward with FP32 adjoint, against the same gradient in FP64; time
@@ -347,6 +399,8 @@ def main():
         res["adjoint_step"] = adjoint_step_rates(w)
     if "adjlarge" in only:
         res["adjoint_large"] = large_adjoint()
+    if "laplacian" in only:
+        res["laplacian"] = laplacian(with_cpu=not a.no_cpu)
 
     parity = None
     try:
