@@ -5,6 +5,7 @@ indices is not periodic), repeated applyFilter/copyOutToIn, checksum and
 VTK output, and the drop-in laplacian executable."""
 from __future__ import annotations
 
+import itertools
 import math
 import os
 import subprocess
@@ -37,9 +38,15 @@ def _rand_stencil(rng, nd, nterms):
     return st
 
 
+def _box27(rng):
+    return {o: float(rng.standard_normal()) for o in itertools.product((-1, 0, 1), repeat=3)}
+
+
+# Grids with axis 1 >= 8 and a stencil ring within 64 KB of LDS take the
+# plane-marching kernel, the others the gather kernel.
 CASES = [((100,), "lap"), ((64, 48), "lap"), ((6, 10), "lap"), ((33, 8, 20), "lap"),
          ((16, 16, 16), "lap"), ((37,), "rand"), ((24, 9), "rand"), ((10, 12, 7), "rand"),
-         ((300, 5, 3), "rand")]
+         ((300, 5, 3), "rand"), ((40, 24, 18), "box"), ((130, 17, 5), "box")]
 
 
 @pytest.mark.parametrize("dims,kind", CASES)
@@ -47,7 +54,8 @@ def test_filter_bit_exact_vs_oracle(oracle, dims, kind):
     from fidibench_amd import Filter
     rng = np.random.default_rng(sum(dims))
     nd = len(dims)
-    st = _lap(nd) if kind == "lap" else _rand_stencil(rng, nd, 9)
+    st = {"lap": lambda: _lap(nd), "rand": lambda: _rand_stencil(rng, nd, 9),
+          "box": lambda: _box27(rng)}[kind]()
     f0 = rng.standard_normal(int(np.prod(dims)))
     flt = Filter(dims, [0.0] * nd, [1.0] * nd, st)
     flt.setInData(f0)
