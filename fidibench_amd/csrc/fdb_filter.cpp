@@ -1,6 +1,6 @@
 // fdb_filter.cpp -- host side of the C ABI for the generic periodic stencil
 // filter (include/fidibench_amd/fdb_filter.h): the one-process form of
-// pletzer/fidibench cxx/Filter.cpp.  The kernels are in fdb_filter.hip.
+// pletzer/fidibench cxx/Filter.cpp.  The kernels are in fdb_filter_kernels.hip.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -17,6 +17,7 @@ struct fdb_filter {
   uint64_t dims[3] = {1, 1, 1};
   double xmins[3] = {0, 0, 0}, xmaxs[3] = {1, 1, 1};
   fdb::FilterGeom g{};
+  fdb::FilterPlan plan{};  // kernel and tiling, fixed at creation
   double* buf[2] = {nullptr, nullptr};
   int in = 0;              // buf[in] holds the input
   bool out_is_in = false;  // after copyOutToIn the output equals the input
@@ -126,7 +127,8 @@ int fdb_filter_create(int ndims, const uint64_t* dims, const double* xmins, cons
     h->xmaxs[a] = xmaxs[a];
   }
   h->g = g;
-  hipError_t e = hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking);
+  hipError_t e = fdb::plan_filter(g, &h->plan);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking);
   for (int i = 0; i < 2 && e == hipSuccess; ++i) {
     e = hipMalloc((void**)&h->buf[i], g.ntot * sizeof(double));
     if (e == hipSuccess) e = hipMemsetAsync(h->buf[i], 0, g.ntot * sizeof(double), h->s);
@@ -167,7 +169,7 @@ int fdb_filter_set_in_sin(fdb_filter_t* h) {
 
 int fdb_filter_apply(fdb_filter_t* h) {
   if (!h) return ffail(FDB_ERR_INVALID, "filter_apply: null handle");
-  FLT_HIP(fdb::launch_filter(h->g, h->buf[h->in], h->buf[1 - h->in], h->s));
+  FLT_HIP(fdb::launch_filter(h->plan, h->buf[h->in], h->buf[1 - h->in], h->s));
   h->out_is_in = false;
   FLT_HIP(hipStreamSynchronize(h->s));
   return FDB_OK;
@@ -186,7 +188,7 @@ int fdb_filter_iterate_timed(fdb_filter_t* h, int niters, float* ms) {
   if (!h || niters < 0) return ffail(FDB_ERR_INVALID, "filter_iterate_timed: bad argument");
   FLT_HIP(hipEventRecord(h->e0, h->s));
   for (int i = 0; i < niters; ++i) {
-    FLT_HIP(fdb::launch_filter(h->g, h->buf[h->in], h->buf[1 - h->in], h->s));
+    FLT_HIP(fdb::launch_filter(h->plan, h->buf[h->in], h->buf[1 - h->in], h->s));
     h->in = 1 - h->in;  // copyOutToIn
     h->out_is_in = true;
   }

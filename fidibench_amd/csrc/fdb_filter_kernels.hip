@@ -73,100 +73,224 @@ __global__ __launch_bounds__(kFilterThreads) void filter_kernel(const double* __
 
 // ---- plane-marching kernel: one block = a TX x TY tile of axes 0/1 over a
 // chunk of axis-2 planes.  The planes z-rz .. z+rz around the output plane
-// sit in an LDS ring, each with an rx/ry halo (loaded through the same
-// wrap, so an LDS value is exactly in[wrapped neighbour]); every cell then
-// sums its terms from LDS in map order.  The next plane's global loads are
-// issued into registers before the current plane is computed.
-constexpr int kTX = 128, kTY = 8;        // tile; 256 threads = 64 x 4
-constexpr int kLoadX = 3, kLoadY = 6;    // LX <= 192, LY <= 24 (|offset| <= 8)
-
-struct MarchTile {
-  int rx, ry, rz, LX, LY, D;
-  int64_t tiles_x, tiles_y, chunk;
-  int lofs[kFilterMaxTerms];  // oy * LX + ox
-  int zo[kFilterMaxTerms];    // oz
+// sit in an LDS ring, each the LX x LY halo tile (rx/ry halos loaded
+// through the same wrap, so an LDS value is exactly in[wrapped
+// neighbour]); every cell then sums its terms from LDS in map order.
+// The halo tile is loaded flat: thread t takes elements t, t + 256, ...
+// (NL per thread, consecutive threads on consecutive x), held in a
+// register ring PF planes deep, so PF planes of loads are in flight while
+// a plane is summed.
+// Block BX x (256/BX) threads, each summing CX x CY cells: TX = BX * CX,
+// TY = (256/BX) * CY.  Shapes: 64 x 4 threads, 2 x 2 cells (128 x 8 tiles)
+// for 3-D grids; 256 x 1 threads, 2 x 1 cells (512 x 1 tiles) for 2-D
+// grids, which march along their axis 1 viewed as axis 2 of an
+// n0 x 1 x n1 grid.
+template <int BX, int CX, int CY, int NL, int PF>
+struct MarchShape {
+  static constexpr int kBX = BX, kBY = 256 / BX, kCX = CX, kCY = CY, kNL = NL, kPF = PF;
+  static constexpr int kTX = BX * CX, kTY = (256 / BX) * CY;
 };
+using March3D = MarchShape<64, 2, 2, 6, 2>;       // halo tile <= 1536 values (radius 1)
+using March3DWide = MarchShape<64, 2, 2, 14, 2>;  // <= 3584 values (radius <= 8)
+using March2D = MarchShape<256, 2, 1, 3, 4>;      // <= 768 values (rx <= 128)
 
-template <int NT>
+// Hardware hands blocks to the 8 XCDs round-robin by block id; renumber
+// them so each XCD takes one contiguous range of tiles (neighbouring tiles,
+// whose halos overlap, then share that XCD's L2).
+__device__ inline int64_t xcd_contiguous(int64_t i, int64_t nb) {
+  const int64_t x = i & 7, q = nb >> 3, r = nb & 7;
+  return x * q + (x < r ? x : r) + (i >> 3);
+}
+
+// Workgroup barrier that orders LDS accesses only: global loads issued
+// before it stay in flight (__syncthreads() would also wait for them).
+__device__ inline void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+// Every memory access of the plane loop is unconditional, so the compiler's
+// wait counts let the prefetched planes stay in flight: lanes outside the
+// halo tile load a valid (wrapped) address and write a spare LDS area of
+// 64 values after each ring plane; output cells past the grid's edge are
+// stored through a raw buffer resource whose range check drops them.
+template <int NT, class S>
 __global__ __launch_bounds__(256) void filter_march_kernel(const double* __restrict__ in,
                                                            double* __restrict__ out,
-                                                           FilterGeom g, MarchTile m) {
+                                                           FilterGeom g, FilterMarch m) {
   extern __shared__ double ring[];
+  constexpr int BX = S::kBX, BY = S::kBY, NL = S::kNL, PF = S::kPF;
   const int tx = threadIdx.x, ty = threadIdx.y;
-  const int64_t b = blockIdx.x;
+  const int tid = ty * BX + tx;
+  const int64_t b = xcd_contiguous(blockIdx.x, gridDim.x);
   const int64_t bx = b % m.tiles_x;
   const int64_t by = (b / m.tiles_x) % m.tiles_y;
   const int64_t bz = b / (m.tiles_x * m.tiles_y);
-  const int64_t x0 = bx * kTX, y0 = by * kTY;
-  const int64_t z0 = bz * m.chunk;
+  const int64_t x0 = bx * S::kTX, y0 = by * S::kTY;
+  const int64_t z0 = bz * m.chunk;  // < n[2]: the grid has ceil(n2 / chunk) chunks
   const int64_t z1 = (z0 + m.chunk < g.n[2]) ? z0 + m.chunk : g.n[2];
-  const int planeLDS = m.LX * m.LY;
+  const int ne = m.LX * m.LY;  // halo-tile values per ring plane
+  const int ps = ne + 64;      // ring plane stride (spare area after each plane)
 
-  // Global offsets (within a plane) of this thread's halo-tile elements.
-  int64_t goff[kLoadY][kLoadX];
-  bool gval[kLoadY][kLoadX];
+  // Global offsets (within a plane) and LDS slots of this thread's
+  // halo-tile elements.
+  int64_t goff[NL];
+  int loff[NL];
 #pragma unroll
-  for (int k = 0; k < kLoadY; ++k) {
-    const int ly = ty + 4 * k;
-    const int64_t yw = filter_wrap(y0 - m.ry + ly, 1, g);
+  for (int k = 0; k < NL; ++k) {
+    const int e = tid + 256 * k;
+    const bool ok = e < ne;
+    const int ly = ok ? e / m.LX : 0;
+    const int lx = ok ? e - ly * m.LX : 0;
+    goff[k] = filter_wrap(x0 - m.rx + lx, 0, g) + g.n[0] * filter_wrap(y0 - m.ry + ly, 1, g);
+    loff[k] = ok ? e : ne + (tid & 63);
+  }
+  // Output offsets (bytes from the tile's first row in a plane); cells
+  // outside the grid get an offset past the buffer's range.
+  int soff[S::kCY][S::kCX];
 #pragma unroll
-    for (int p = 0; p < kLoadX; ++p) {
-      const int lx = tx + 64 * p;
-      gval[k][p] = ly < m.LY && lx < m.LX;
-      goff[k][p] = filter_wrap(x0 - m.rx + lx, 0, g) + g.n[0] * yw;
+  for (int j = 0; j < S::kCY; ++j)
+#pragma unroll
+    for (int i = 0; i < S::kCX; ++i) {
+      const int64_t x = x0 + tx + BX * i, y = y0 + ty + BY * j;
+      soff[j][i] = (x < g.n[0] && y < g.n[1]) ? (int)((x + g.n[0] * (y - y0)) * 8) : 0x7fffffff;
     }
-  }
-  auto slot = [&](int64_t q) { return (int)(((q % m.D) + m.D) % m.D); };
-  auto fetch = [&](int64_t q, double (&v)[kLoadY][kLoadX]) {
-    const double* pl = in + filter_wrap(q, 2, g) * g.n[0] * g.n[1];
-#pragma unroll
-    for (int k = 0; k < kLoadY; ++k)
-#pragma unroll
-      for (int p = 0; p < kLoadX; ++p)
-        if (gval[k][p]) v[k][p] = pl[goff[k][p]];
-  };
-  auto put = [&](int64_t q, const double (&v)[kLoadY][kLoadX]) {
-    double* dst = ring + slot(q) * planeLDS;
-#pragma unroll
-    for (int k = 0; k < kLoadY; ++k)
-#pragma unroll
-      for (int p = 0; p < kLoadX; ++p)
-        if (gval[k][p]) dst[(ty + 4 * k) * m.LX + tx + 64 * p] = v[k][p];
-  };
+  const int64_t plane = g.n[0] * g.n[1];
+  const int64_t tile_bytes = (g.n[1] - y0) * g.n[0] * 8;  // tile's first row to the plane's end
+  const int nrec = (int)(tile_bytes < 0x7fffffffLL ? tile_bytes : 0x7fffffffLL);
 
-  double v[kLoadY][kLoadX];
-  for (int64_t q = z0 - m.rz; q < z0 + m.rz; ++q) {  // warm-up planes
-    fetch(q, v);
-    put(q, v);
-  }
-  if (z0 < z1) fetch(z0 + m.rz, v);
+  // Ring slots: cur is the slot of the output plane, advanced by one per
+  // plane; plane cur + d (|d| <= 2 rz) wraps once at most.
+  auto wrapD = [&](int x) { return x < 0 ? x + m.D : (x >= m.D ? x - m.D : x); };
+  auto fetch = [&](int64_t q, double (&v)[NL]) {
+    const double* pl = in + filter_wrap(q, 2, g) * plane;
+#pragma unroll
+    for (int k = 0; k < NL; ++k) v[k] = pl[goff[k]];
+  };
+  auto put = [&](int sl, const double (&v)[NL]) {
+    double* dst = ring + sl * ps;
+#pragma unroll
+    for (int k = 0; k < NL; ++k) dst[loff[k]] = v[k];
+  };
   const int nt = NT > 0 ? NT : g.nt;
-  for (int64_t z = z0; z < z1; ++z) {
-    put(z + m.rz, v);
-    __syncthreads();
-    if (z + 1 < z1) fetch(z + 1 + m.rz, v);  // in flight during the sums
-    double* ob = out + z * g.n[0] * g.n[1];
+  auto sums = [&](int64_t z, int cur) {
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(out + z * plane + y0 * g.n[0], 0, nrec, 0x00020000);
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int64_t y = y0 + ty + 4 * j;
+    for (int j = 0; j < S::kCY; ++j) {
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        const int64_t x = x0 + tx + 64 * i;
-        const int base = (ty + 4 * j + m.ry) * m.LX + tx + 64 * i + m.rx;
+      for (int i = 0; i < S::kCX; ++i) {
+        const int base = (ty + BY * j + m.ry) * m.LX + tx + BX * i + m.rx;
         double acc = 0.0;
         if constexpr (NT > 0) {
 #pragma unroll
           for (int t = 0; t < NT; ++t)
-            acc = acc + g.w[t] * ring[slot(z + m.zo[t]) * planeLDS + base + m.lofs[t]];
+            acc = acc + g.w[t] * ring[wrapD(cur + m.zo[t]) * ps + base + m.lofs[t]];
         } else {
           for (int t = 0; t < nt; ++t)
-            acc = acc + g.w[t] * ring[slot(z + m.zo[t]) * planeLDS + base + m.lofs[t]];
+            acc = acc + g.w[t] * ring[wrapD(cur + m.zo[t]) * ps + base + m.lofs[t]];
         }
-        if (x < g.n[0] && y < g.n[1]) __builtin_nontemporal_store(acc, ob + x + g.n[0] * y);
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, acc), rs, soff[j][i], 0,
+                                              2 /* nt */);
       }
     }
-    __syncthreads();  // the next put overwrites the oldest plane
+  };
+
+  double v[PF][NL];
+  int cur = m.rz;  // slot of plane z0; plane z0 - rz takes slot 0
+  for (int d = -m.rz; d < m.rz; ++d) {  // warm-up planes
+    fetch(z0 + d, v[0]);
+    put(cur + d, v[0]);
   }
+#pragma unroll
+  for (int u = 0; u < PF; ++u) fetch((z0 + u < z1 ? z0 + u : z1 - 1) + m.rz, v[u]);
+  // One plane: store plane zz + rz (prefetched), refill its registers with
+  // plane zz + PF + rz (clamped to the chunk: a re-read, not a branch),
+  // sum plane zz.
+  auto step = [&](int64_t zz, double (&vu)[NL]) {
+    put(wrapD(cur + m.rz), vu);
+    lds_barrier();
+    fetch((zz + PF < z1 ? zz + PF : z1 - 1) + m.rz, vu);
+    sums(zz, cur);
+    cur = cur + 1 == m.D ? 0 : cur + 1;
+    lds_barrier();  // the next put overwrites the oldest plane
+  };
+  int64_t z = z0;
+  for (; z + PF <= z1; z += PF) {
+#pragma unroll
+    for (int u = 0; u < PF; ++u) step(z + u, v[u]);
+  }
+#pragma unroll
+  for (int u = 0; u < PF - 1; ++u)
+    if (z + u < z1) step(z + u, v[u]);  // block-uniform
+}
+
+template <int NT, class S>
+const void* march_kernel_ptr() {
+  return reinterpret_cast<const void*>(&filter_march_kernel<NT, S>);
+}
+
+template <class S>
+const void* march_kernel_for(int nt) {
+  return nt == 5 ? march_kernel_ptr<5, S>() : nt == 7 ? march_kernel_ptr<7, S>()
+                                                        : march_kernel_ptr<0, S>();
+}
+
+// Radii, ring size and tiling of the marching kernel for geometry g and
+// shape S; false if the ring does not fit in 64 KB of LDS or the halo
+// tile exceeds the shape's loads (256 x NL values).  The axis-2 chunk count fills
+// the device's resident blocks in whole rounds, counting the 2 rz warm-up
+// planes a chunk reads before its first output plane.
+template <class S>
+bool march_plan(const FilterGeom& g, FilterMarch& m, size_t& lds, int64_t& blocks) {
+  m = FilterMarch{};
+  for (int t = 0; t < g.nt; ++t) {
+    m.rx = std::max(m.rx, std::abs(g.off[t][0]));
+    m.ry = std::max(m.ry, std::abs(g.off[t][1]));
+    m.rz = std::max(m.rz, std::abs(g.off[t][2]));
+  }
+  m.LX = S::kTX + 2 * m.rx;
+  m.LY = S::kTY + 2 * m.ry;
+  m.D = 2 * m.rz + 1;
+  lds = (size_t)m.D * (m.LX * m.LY + 64) * sizeof(double);
+  if (m.LX * m.LY > 256 * S::kNL || lds > 65536) return false;
+  // store offsets within one tile (TY rows of a plane) fit in 31 bits
+  if ((int64_t)S::kTY * g.n[0] * 8 >= 0x7fffffffLL) return false;
+  for (int t = 0; t < g.nt; ++t) {
+    m.lofs[t] = g.off[t][1] * m.LX + g.off[t][0];
+    m.zo[t] = g.off[t][2];
+  }
+  m.tiles_x = (g.n[0] + S::kTX - 1) / S::kTX;
+  m.tiles_y = (g.n[1] + S::kTY - 1) / S::kTY;
+  const int64_t tiles = m.tiles_x * m.tiles_y;
+  const int64_t cap = kernel_capacity(march_kernel_for<S>(g.nt), 256, lds);
+  const int64_t per_round = std::max<int64_t>(1, cap / tiles);
+  const int64_t nch =
+      march_chunks(tiles, g.n[2], 2 * m.rz, cap, std::max<int64_t>(1, per_round / 2),
+                   4 * per_round, true);
+  m.chunk = (g.n[2] + nch - 1) / nch;
+  blocks = tiles * ((g.n[2] + m.chunk - 1) / m.chunk);
+  return blocks > 0 && blocks <= 0x7fffffffLL;
+}
+
+template <class S>
+hipError_t launch_march(const FilterPlan& p, const double* in, double* out, hipStream_t s) {
+  const dim3 grid((unsigned)p.blocks), block(S::kBX, S::kBY);
+  switch (p.g.nt) {
+    case 5:
+      hipLaunchKernelGGL((filter_march_kernel<5, S>), grid, block, p.lds, s, in, out, p.g, p.m);
+      break;
+    case 7:
+      hipLaunchKernelGGL((filter_march_kernel<7, S>), grid, block, p.lds, s, in, out, p.g, p.m);
+      break;
+    default:
+      hipLaunchKernelGGL((filter_march_kernel<0, S>), grid, block, p.lds, s, in, out, p.g, p.m);
+      break;
+  }
+  return hipGetLastError();
 }
 
 struct SinArgs {
@@ -192,44 +316,58 @@ __global__ __launch_bounds__(kFilterThreads) void filter_sin_kernel(double* __re
 
 }  // namespace
 
-hipError_t launch_filter(const FilterGeom& g, const double* in, double* out, hipStream_t s) {
-  // The marching kernel for 2-D/3-D grids whose stencil ring fits in 64 KB
-  // of LDS; the gather kernel otherwise (1-D grids, wide stencils).
-  MarchTile m = {};
-  for (int t = 0; t < g.nt; ++t) {
-    m.rx = std::max(m.rx, std::abs(g.off[t][0]));
-    m.ry = std::max(m.ry, std::abs(g.off[t][1]));
-    m.rz = std::max(m.rz, std::abs(g.off[t][2]));
-  }
-  m.LX = kTX + 2 * m.rx;
-  m.LY = kTY + 2 * m.ry;
-  m.D = 2 * m.rz + 1;
-  const size_t lds = (size_t)m.D * m.LX * m.LY * sizeof(double);
-  if (g.n[1] >= kTY && lds <= 65536) {
+hipError_t plan_filter(const FilterGeom& g, FilterPlan* p) {
+  // 2-D grids (no axis-2 extent or offsets): rows marched along axis 1,
+  // the grid viewed as n0 x 1 x n1 (same memory order).  3-D grids with
+  // n1 >= 8: planes marched along axis 2.  Otherwise (1-D grids, stencils
+  // wider than the ring allows): the gather kernel.
+  *p = FilterPlan{};
+  bool flat = g.n[2] == 1 && g.n[1] > 1;
+  for (int t = 0; t < g.nt && flat; ++t) flat = g.off[t][2] == 0;
+  if (flat) {
+    FilterGeom h = g;
+    h.n[1] = 1;
+    h.n[2] = g.n[1];
     for (int t = 0; t < g.nt; ++t) {
-      m.lofs[t] = g.off[t][1] * m.LX + g.off[t][0];
-      m.zo[t] = g.off[t][2];
+      h.off[t][1] = 0;
+      h.off[t][2] = g.off[t][1];
     }
-    m.tiles_x = (g.n[0] + kTX - 1) / kTX;
-    m.tiles_y = (g.n[1] + kTY - 1) / kTY;
-    const int64_t tiles = m.tiles_x * m.tiles_y;
-    // about 2048 blocks (8 per CU), each marching a chunk of planes
-    const int64_t nch = std::max<int64_t>(1, std::min<int64_t>(g.n[2], (2048 + tiles - 1) / tiles));
-    m.chunk = (g.n[2] + nch - 1) / nch;
-    const int64_t nb = tiles * ((g.n[2] + m.chunk - 1) / m.chunk);
-    if (nb <= 0 || nb > 0x7fffffffLL) return hipErrorInvalidConfiguration;
-    const dim3 grid((unsigned)nb), block(64, 4);
-    switch (g.nt) {
-      case 5: hipLaunchKernelGGL(filter_march_kernel<5>, grid, block, lds, s, in, out, g, m); break;
-      case 7: hipLaunchKernelGGL(filter_march_kernel<7>, grid, block, lds, s, in, out, g, m); break;
-      default: hipLaunchKernelGGL(filter_march_kernel<0>, grid, block, lds, s, in, out, g, m); break;
+    for (int k = 0; k < kFilterMaxOffset; ++k) {
+      h.neg[1][k] = 0;
+      h.neg[2][k] = g.neg[1][k];
     }
-    return hipGetLastError();
+    if (march_plan<March2D>(h, p->m, p->lds, p->blocks)) {
+      p->kind = kFilterRows;
+      p->g = h;
+      return hipSuccess;
+    }
   }
+  if (g.n[1] >= March3D::kTY && march_plan<March3D>(g, p->m, p->lds, p->blocks)) {
+    p->kind = kFilterPlanes;
+    p->g = g;
+    return hipSuccess;
+  }
+  if (g.n[1] >= March3DWide::kTY && march_plan<March3DWide>(g, p->m, p->lds, p->blocks)) {
+    p->kind = kFilterPlanesWide;
+    p->g = g;
+    return hipSuccess;
+  }
+  p->kind = kFilterGather;
+  p->g = g;
+  p->lds = 0;
   const int64_t xblocks = (g.n[0] + kFilterThreads - 1) / kFilterThreads;
-  const int64_t nb = xblocks * g.n[1] * g.n[2];
-  if (nb <= 0 || nb > 0x7fffffffLL) return hipErrorInvalidConfiguration;
-  const dim3 grid((unsigned)nb), block(kFilterThreads);
+  p->blocks = xblocks * g.n[1] * g.n[2];
+  if (p->blocks <= 0 || p->blocks > 0x7fffffffLL) return hipErrorInvalidConfiguration;
+  return hipSuccess;
+}
+
+hipError_t launch_filter(const FilterPlan& p, const double* in, double* out, hipStream_t s) {
+  if (p.kind == kFilterRows) return launch_march<March2D>(p, in, out, s);
+  if (p.kind == kFilterPlanes) return launch_march<March3D>(p, in, out, s);
+  if (p.kind == kFilterPlanesWide) return launch_march<March3DWide>(p, in, out, s);
+  const FilterGeom& g = p.g;
+  const int64_t xblocks = (g.n[0] + kFilterThreads - 1) / kFilterThreads;
+  const dim3 grid((unsigned)p.blocks), block(kFilterThreads);
   switch (g.nt) {
     case 3: hipLaunchKernelGGL(filter_kernel<3>, grid, block, 0, s, in, out, g, xblocks); break;
     case 5: hipLaunchKernelGGL(filter_kernel<5>, grid, block, 0, s, in, out, g, xblocks); break;

@@ -64,7 +64,16 @@ hipError_t launch_fused_pass(int dtype, const Grid3& g, const void* src, void* d
 // The storage layout of a dtype (FDB_BF16_SR is stored as FDB_BF16).
 inline int storage_dtype(int dtype) { return dtype == 3 ? 2 : dtype; }
 
-// ---- generic periodic stencil (cxx/Filter.cpp), kernels in fdb_filter.hip
+// Launch sizing shared by the marching kernels.
+int device_cus();
+// Blocks of kernel `kern` (host stub) with `threads` threads and `dyn_lds`
+// bytes of dynamic LDS that the device holds at once.
+int64_t kernel_capacity(const void* kern, int threads, size_t dyn_lds);
+// Chunk count for a march over n0 planes (fdb_kernels.hip).
+int64_t march_chunks(int64_t tiles, int64_t n0, int extra, int64_t capacity, int64_t lo,
+                     int64_t hi, bool fill);
+
+// ---- generic periodic stencil (cxx/Filter.cpp), kernels in fdb_filter_kernels.hip
 
 constexpr int kFilterMaxTerms = 32;   // stencil terms
 constexpr int kFilterMaxOffset = 8;   // |offset| per axis
@@ -83,8 +92,34 @@ struct FilterGeom {
   int64_t neg[3][kFilterMaxOffset];
 };
 
+// Tiling of the marching kernels: a TX x TY tile of axes 0/1 with rx/ry
+// halos (LX x LY values per plane), a ring of D = 2 rz + 1 planes, tiles
+// of `chunk` consecutive axis-2 planes each.
+struct FilterMarch {
+  int rx, ry, rz, LX, LY, D;
+  int64_t tiles_x, tiles_y, chunk;
+  int lofs[kFilterMaxTerms];  // oy * LX + ox, per term
+  int zo[kFilterMaxTerms];    // oz, per term
+};
+
+// How one filter application is launched, fixed once per geometry.
+struct FilterPlan {
+  int kind;      // kFilterGather, kFilterPlanes(Wide) or kFilterRows
+  FilterGeom g;  // geometry as launched (kFilterRows: viewed as n0 x 1 x n1)
+  FilterMarch m;
+  size_t lds;    // dynamic LDS bytes per block
+  int64_t blocks;
+};
+constexpr int kFilterGather = 0;  // one thread per output cell, neighbours from HBM/L2
+constexpr int kFilterPlanes = 1;  // 3-D: 128 x 8 tiles marched along axis 2
+constexpr int kFilterRows = 2;    // 2-D: 512-wide tiles marched along axis 1
+constexpr int kFilterPlanesWide = 3;  // 3-D, halo tiles of radius 2..8
+
+// Picks the kernel and tiling for g (needs the device current).
+hipError_t plan_filter(const FilterGeom& g, FilterPlan* p);
+
 // out[i] = sum over terms, in order, of w * in[wrapped neighbour].
-hipError_t launch_filter(const FilterGeom& g, const double* in, double* out, hipStream_t s);
+hipError_t launch_filter(const FilterPlan& p, const double* in, double* out, hipStream_t s);
 
 // in[i] = prod_a sin(2 pi x_a), x_a = xmin_a + (i_a + 0.5) * (xmax_a - xmin_a) / n_a
 // (the input function of laplacian/cxx/laplacian.cxx:22-28), with the

@@ -917,6 +917,9 @@ struct RowsCfg {
 
 constexpr int64_t kTargetBlocks = 1024;
 
+}  // namespace
+
+// The device's compute units (queried once).
 int device_cus() {
   static int cus = [] {
     int dev = 0, n = 0;
@@ -928,24 +931,25 @@ int device_cus() {
   return cus;
 }
 
-// Blocks of `kern` (with `threads` threads) the device holds at once.
-template <typename Kern>
-int64_t resident_blocks(Kern kern, int threads) {
+int64_t kernel_capacity(const void* kern, int threads, size_t dyn_lds) {
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, 0) != hipSuccess ||
-      per_cu < 1)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, dyn_lds) !=
+      hipSuccess) {
+    (void)hipGetLastError();  // a failed query only loses the estimate
     per_cu = 1;
-  return (int64_t)per_cu * device_cus();
+  }
+  return (int64_t)std::max(per_cu, 1) * device_cus();
 }
 
-// Number of axis-0 chunks, between lo and hi, for `tiles` tiles per chunk:
-// every block is one tile, the device holds `capacity` blocks at once, and
-// a chunk of c planes costs c + extra plane visits (planes read before the
-// first output plane).  Picks the count that wastes least on the last,
-// partly filled round of blocks and on the extra planes.  With
-// fill = false a single, partly filled round counts as full (a kernel
-// that saturates HBM with fewer resident blocks); with fill = true it
-// counts by its fill (a kernel whose rate scales with resident waves).
+// Number of chunks of the marched axis, between lo and hi, for `tiles`
+// tiles per chunk: every block is one tile, the device holds `capacity`
+// blocks at once, and a chunk of c planes costs c + extra plane visits
+// (planes read before the first output plane).  Picks the count that
+// wastes least on the last, partly filled round of blocks and on the extra
+// planes.  With fill = false a single, partly filled round counts as full
+// (a kernel that saturates HBM with fewer resident blocks); with
+// fill = true it counts by its fill (a kernel whose rate scales with
+// resident waves).
 int64_t march_chunks(int64_t tiles, int64_t n0, int extra, int64_t capacity, int64_t lo,
                      int64_t hi, bool fill) {
   int64_t best = 1;
@@ -964,6 +968,15 @@ int64_t march_chunks(int64_t tiles, int64_t n0, int extra, int64_t capacity, int
     }
   }
   return best;
+}
+
+namespace {
+
+// Blocks of `kern` (with `threads` threads, static LDS only) the device
+// holds at once.
+template <typename Kern>
+int64_t resident_blocks(Kern kern, int threads) {
+  return kernel_capacity(reinterpret_cast<const void*>(kern), threads, 0);
 }
 
 RowsCfg parse_cfg(const char* s, RowsCfg c) {

@@ -42,11 +42,14 @@ def _box27(rng):
     return {o: float(rng.standard_normal()) for o in itertools.product((-1, 0, 1), repeat=3)}
 
 
-# Grids with axis 1 >= 8 and a stencil ring within 64 KB of LDS take the
-# plane-marching kernel, the others the gather kernel.
+# 2-D grids take the row-marching kernel (512-wide tiles; 1030 x 21 has a
+# partial third tile), 3-D grids with axis 1 >= 8 the plane-marching one
+# (40 x 30 x 1 with axis-2 offsets that all wrap onto plane 0), the others
+# (1-D, 300 x 5 x 3) the gather kernel.
 CASES = [((100,), "lap"), ((64, 48), "lap"), ((6, 10), "lap"), ((33, 8, 20), "lap"),
          ((16, 16, 16), "lap"), ((37,), "rand"), ((24, 9), "rand"), ((10, 12, 7), "rand"),
-         ((300, 5, 3), "rand"), ((40, 24, 18), "box"), ((130, 17, 5), "box")]
+         ((300, 5, 3), "rand"), ((40, 24, 18), "box"), ((130, 17, 5), "box"),
+         ((1030, 21), "rand"), ((40, 30, 1), "lap")]
 
 
 @pytest.mark.parametrize("dims,kind", CASES)
