@@ -272,8 +272,17 @@ def laplacian(iters=10, with_cpu=True):
         ms = flt.iterate(iters)
         ntot = n ** nd
         cu = ntot * iters / (ms * 1e-3)
+        launch_ms = ms / iters  # one kernel launch per iteration (copyOutToIn swaps buffers)
+        achieved = 16 * ntot / (launch_ms * 1e-3) / 1e9
         entry = {"cells": ntot, "iterations": iters, "ms": ms, "cell_updates_per_s": cu,
                  "hbm_frac_compulsory": cu * 16 / HBM_PEAK,
+                 "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9,
+                              "unit": "GB/s", "frac": achieved * 1e9 / HBM_PEAK,
+                              "traffic": pmc_traffic(name,
+                                                     "filter_march_kernel_laplacian_stats.md"),
+                              "kernel": ("filter_march_kernel (rows, 2-D)" if nd == 2 else
+                                         "filter_march_kernel (planes, 3-D)"),
+                              "avg_launch_ms": launch_ms},
                  "checksum_output": flt.computeCheckSum("output")}
         if name == "lap2d_8000" and with_cpu:
             try:
@@ -333,9 +342,9 @@ def large_adjoint(shape=(1024, 1024, 768), steps=400):
             "grad_v_fp64": gv64}
 
 
-def pmc_traffic(kernel_key):
-    """HBM bytes per launch from the committed rocprofv3 PMC summary, if any."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic.txt")
+def pmc_traffic(kernel_key, summary="pmc_traffic.txt"):
+    """HBM bytes per launch from a committed rocprofv3 PMC summary, if any."""
+    path = os.path.join(ROOT, "profiles", summary)
     try:
         for line in open(path):
             m = re.match(r"\s*" + re.escape(kernel_key) + r"\s*:\s*([0-9.eE+]+)", line)
