@@ -91,6 +91,7 @@ struct MarchShape {
   static constexpr int kTX = BX * CX, kTY = (256 / BX) * CY;
 };
 using March3D = MarchShape<64, 2, 2, 6, 2>;       // halo tile <= 1536 values (radius 1)
+using March3DMid = MarchShape<64, 2, 1, 6, 2>;    // 128 x 4 tiles, <= 1536 (radius 2..3)
 using March3DWide = MarchShape<64, 2, 2, 14, 2>;  // <= 3584 values (radius <= 8)
 using March2D = MarchShape<256, 2, 1, 3, 4>;      // <= 768 values (rx <= 128)
 
@@ -233,16 +234,29 @@ const void* march_kernel_ptr() {
   return reinterpret_cast<const void*>(&filter_march_kernel<NT, S>);
 }
 
+// Term counts with an unrolled kernel: 2nd-order Laplacians (5, 7),
+// 4th-order ones (9, 13), the 27-point box; others loop over g.nt.
 template <class S>
 const void* march_kernel_for(int nt) {
-  return nt == 5 ? march_kernel_ptr<5, S>() : nt == 7 ? march_kernel_ptr<7, S>()
-                                                        : march_kernel_ptr<0, S>();
+  switch (nt) {
+    case 5: return march_kernel_ptr<5, S>();
+    case 7: return march_kernel_ptr<7, S>();
+    case 9: return march_kernel_ptr<9, S>();
+    case 13: return march_kernel_ptr<13, S>();
+    case 27: return march_kernel_ptr<27, S>();
+    default: return march_kernel_ptr<0, S>();
+  }
 }
 
+// LDS of one workgroup on gfx950; rings above the default 64 KB of dynamic
+// LDS need the kernel's limit raised (hipFuncSetAttribute).
+constexpr size_t kMaxRingLds = 160 * 1024;
+
 // Radii, ring size and tiling of the marching kernel for geometry g and
-// shape S; false if the ring does not fit in 64 KB of LDS or the halo
-// tile exceeds the shape's loads (256 x NL values).  The axis-2 chunk count fills
-// the device's resident blocks in whole rounds, counting the 2 rz warm-up
+// shape S; false if the ring does not fit in the workgroup's LDS (or the
+// runtime refuses to raise the kernel's limit) or the halo tile exceeds
+// the shape's loads (256 x NL values).  The axis-2 chunk count fills the
+// device's resident blocks in whole rounds, counting the 2 rz warm-up
 // planes a chunk reads before its first output plane.
 template <class S>
 bool march_plan(const FilterGeom& g, FilterMarch& m, size_t& lds, int64_t& blocks) {
@@ -256,7 +270,7 @@ bool march_plan(const FilterGeom& g, FilterMarch& m, size_t& lds, int64_t& block
   m.LY = S::kTY + 2 * m.ry;
   m.D = 2 * m.rz + 1;
   lds = (size_t)m.D * (m.LX * m.LY + 64) * sizeof(double);
-  if (m.LX * m.LY > 256 * S::kNL || lds > 65536) return false;
+  if (m.LX * m.LY > 256 * S::kNL || lds > kMaxRingLds) return false;
   // store offsets within one tile (TY rows of a plane) fit in 31 bits
   if ((int64_t)S::kTY * g.n[0] * 8 >= 0x7fffffffLL) return false;
   for (int t = 0; t < g.nt; ++t) {
@@ -266,7 +280,14 @@ bool march_plan(const FilterGeom& g, FilterMarch& m, size_t& lds, int64_t& block
   m.tiles_x = (g.n[0] + S::kTX - 1) / S::kTX;
   m.tiles_y = (g.n[1] + S::kTY - 1) / S::kTY;
   const int64_t tiles = m.tiles_x * m.tiles_y;
-  const int64_t cap = kernel_capacity(march_kernel_for<S>(g.nt), 256, lds);
+  const void* kern = march_kernel_for<S>(g.nt);
+  if (lds > 65536 &&
+      hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+          hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  const int64_t cap = kernel_capacity(kern, 256, lds);
   const int64_t per_round = std::max<int64_t>(1, cap / tiles);
   const int64_t nch =
       march_chunks(tiles, g.n[2], 2 * m.rz, cap, std::max<int64_t>(1, per_round / 2),
@@ -280,12 +301,16 @@ template <class S>
 hipError_t launch_march(const FilterPlan& p, const double* in, double* out, hipStream_t s) {
   const dim3 grid((unsigned)p.blocks), block(S::kBX, S::kBY);
   switch (p.g.nt) {
-    case 5:
-      hipLaunchKernelGGL((filter_march_kernel<5, S>), grid, block, p.lds, s, in, out, p.g, p.m);
-      break;
-    case 7:
-      hipLaunchKernelGGL((filter_march_kernel<7, S>), grid, block, p.lds, s, in, out, p.g, p.m);
-      break;
+#define FDB_MARCH_NT(N)                                                                      \
+  case N:                                                                                    \
+    hipLaunchKernelGGL((filter_march_kernel<N, S>), grid, block, p.lds, s, in, out, p.g, p.m); \
+    break;
+    FDB_MARCH_NT(5)
+    FDB_MARCH_NT(7)
+    FDB_MARCH_NT(9)
+    FDB_MARCH_NT(13)
+    FDB_MARCH_NT(27)
+#undef FDB_MARCH_NT
     default:
       hipLaunchKernelGGL((filter_march_kernel<0, S>), grid, block, p.lds, s, in, out, p.g, p.m);
       break;
@@ -319,8 +344,9 @@ __global__ __launch_bounds__(kFilterThreads) void filter_sin_kernel(double* __re
 hipError_t plan_filter(const FilterGeom& g, FilterPlan* p) {
   // 2-D grids (no axis-2 extent or offsets): rows marched along axis 1,
   // the grid viewed as n0 x 1 x n1 (same memory order).  3-D grids with
-  // n1 >= 8: planes marched along axis 2.  Otherwise (1-D grids, stencils
-  // wider than the ring allows): the gather kernel.
+  // n1 >= 8: planes marched along axis 2 (rings up to 160 KB of LDS).
+  // Otherwise (1-D grids, stencils wider than the ring allows): the
+  // gather kernel.
   *p = FilterPlan{};
   bool flat = g.n[2] == 1 && g.n[1] > 1;
   for (int t = 0; t < g.nt && flat; ++t) flat = g.off[t][2] == 0;
@@ -347,6 +373,11 @@ hipError_t plan_filter(const FilterGeom& g, FilterPlan* p) {
     p->g = g;
     return hipSuccess;
   }
+  if (g.n[1] >= March3DMid::kTY && march_plan<March3DMid>(g, p->m, p->lds, p->blocks)) {
+    p->kind = kFilterPlanesMid;
+    p->g = g;
+    return hipSuccess;
+  }
   if (g.n[1] >= March3DWide::kTY && march_plan<March3DWide>(g, p->m, p->lds, p->blocks)) {
     p->kind = kFilterPlanesWide;
     p->g = g;
@@ -364,6 +395,7 @@ hipError_t plan_filter(const FilterGeom& g, FilterPlan* p) {
 hipError_t launch_filter(const FilterPlan& p, const double* in, double* out, hipStream_t s) {
   if (p.kind == kFilterRows) return launch_march<March2D>(p, in, out, s);
   if (p.kind == kFilterPlanes) return launch_march<March3D>(p, in, out, s);
+  if (p.kind == kFilterPlanesMid) return launch_march<March3DMid>(p, in, out, s);
   if (p.kind == kFilterPlanesWide) return launch_march<March3DWide>(p, in, out, s);
   const FilterGeom& g = p.g;
   const int64_t xblocks = (g.n[0] + kFilterThreads - 1) / kFilterThreads;

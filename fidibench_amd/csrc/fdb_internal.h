@@ -104,7 +104,7 @@ struct FilterMarch {
 
 // How one filter application is launched, fixed once per geometry.
 struct FilterPlan {
-  int kind;      // kFilterGather, kFilterPlanes(Wide) or kFilterRows
+  int kind;      // kFilterGather, kFilterPlanes(Mid, Wide) or kFilterRows
   FilterGeom g;  // geometry as launched (kFilterRows: viewed as n0 x 1 x n1)
   FilterMarch m;
   size_t lds;    // dynamic LDS bytes per block
@@ -113,7 +113,8 @@ struct FilterPlan {
 constexpr int kFilterGather = 0;  // one thread per output cell, neighbours from HBM/L2
 constexpr int kFilterPlanes = 1;  // 3-D: 128 x 8 tiles marched along axis 2
 constexpr int kFilterRows = 2;    // 2-D: 512-wide tiles marched along axis 1
-constexpr int kFilterPlanesWide = 3;  // 3-D, halo tiles of radius 2..8
+constexpr int kFilterPlanesWide = 3;  // 3-D, halo tiles up to radius 8
+constexpr int kFilterPlanesMid = 4;   // 3-D, 128 x 4 tiles, radius 2..3
 
 // Picks the kernel and tiling for g (needs the device current).
 hipError_t plan_filter(const FilterGeom& g, FilterPlan* p);

@@ -38,18 +38,32 @@ def _rand_stencil(rng, nd, nterms):
     return st
 
 
+def _lap4(nd):
+    """Fourth-order Laplacian: radius 2 along every axis."""
+    st = {tuple([0] * nd): -30.0 / 12.0 * nd}
+    for a in range(nd):
+        for s, w in ((1, 16.0 / 12.0), (2, -1.0 / 12.0)):
+            for sg in (1, -1):
+                o = [0] * nd
+                o[a] = sg * s
+                st[tuple(o)] = w
+    return st
+
+
 def _box27(rng):
     return {o: float(rng.standard_normal()) for o in itertools.product((-1, 0, 1), repeat=3)}
 
 
 # 2-D grids take the row-marching kernel (512-wide tiles; 1030 x 21 has a
-# partial third tile), 3-D grids with axis 1 >= 8 the plane-marching one
-# (40 x 30 x 1 with axis-2 offsets that all wrap onto plane 0), the others
-# (1-D, 300 x 5 x 3) the gather kernel.
+# partial third tile), 3-D grids the plane-marching one: 128 x 8 tiles for
+# radius 1 (40 x 30 x 1 with axis-2 offsets that all wrap onto plane 0),
+# 128 x 4 tiles for radius 2-3 (rings of 45-79 KB of LDS; 300 x 5 x 3 has
+# axis 1 shorter than a tile); 1-D grids the gather kernel.
 CASES = [((100,), "lap"), ((64, 48), "lap"), ((6, 10), "lap"), ((33, 8, 20), "lap"),
          ((16, 16, 16), "lap"), ((37,), "rand"), ((24, 9), "rand"), ((10, 12, 7), "rand"),
          ((300, 5, 3), "rand"), ((40, 24, 18), "box"), ((130, 17, 5), "box"),
-         ((1030, 21), "rand"), ((40, 30, 1), "lap")]
+         ((1030, 21), "rand"), ((40, 30, 1), "lap"), ((40, 24, 30), "lap4"),
+         ((70, 9, 11), "lap4"), ((300, 40), "lap4")]
 
 
 @pytest.mark.parametrize("dims,kind", CASES)
@@ -58,7 +72,7 @@ def test_filter_bit_exact_vs_oracle(oracle, dims, kind):
     rng = np.random.default_rng(sum(dims))
     nd = len(dims)
     st = {"lap": lambda: _lap(nd), "rand": lambda: _rand_stencil(rng, nd, 9),
-          "box": lambda: _box27(rng)}[kind]()
+          "box": lambda: _box27(rng), "lap4": lambda: _lap4(nd)}[kind]()
     f0 = rng.standard_normal(int(np.prod(dims)))
     flt = Filter(dims, [0.0] * nd, [1.0] * nd, st)
     flt.setInData(f0)
