@@ -59,6 +59,7 @@ SIGNATURES = {
     "fdb_tape_forward": (_i, [_vp, _vp, _vp, _vp]),
     "fdb_tape_backward": (_i, [_vp, _vp, _vp, _dp, _vp]),
     "fdb_tape_stats": (_i, [_vp, C.POINTER(_i), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "fdb_tape_set_time_block": (_i, [_vp, _i]),
     "fdb_tape_destroy": (_i, [_vp]),
     "fdb_init_field_dev": (_i, [_i, _u64p, _dp, _i, _i, _dp, _i, C.c_uint64, _vp, _vp]),
     "fdb_compare_dev": (_i, [_i, _vp, _i, _vp, C.c_uint64, _dp, _vp]),

@@ -54,6 +54,11 @@ class Tape:
                                           _stream()))
         return [gv[i] for i in range(self.ndims)]
 
+    def set_time_block(self, k: int) -> None:
+        """Steps per launch (1..8): fused forward runs and, where every state
+        is stored, fused 4-step adjoint runs; 1 = one step per launch."""
+        L.check(L.lib().fdb_tape_set_time_block(self._h, int(k)))
+
     def stats(self) -> dict:
         s, f, a = C.c_int(), C.c_int64(), C.c_int64()
         L.check(L.lib().fdb_tape_stats(self._h, C.byref(s), C.byref(f), C.byref(a)))

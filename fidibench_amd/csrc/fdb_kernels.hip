@@ -765,16 +765,29 @@ struct TbArgs {
   int64_t tiles2, tiles1, ntiles;
 };
 
+// PROD = true: the fused transpose (adjoint) pass.  The kernel is called
+// with the adjoint's directions (-u), so level l+1 = A^T (level l), and
+// fs[l] is the forward state of that adjoint step (level l of the pass
+// consumes f_{t-l} when the pass starts from lam_{t+1}).  At every output
+// cell it also sums f[i] * (lam[i + dir_j e_j] - lam[i]) for j = 0, 1, 2 --
+// the one-step adjoint's velocity products -- in FP64 (differences taken
+// in FP64 as there), Neumaier-compensated per thread and reduced per block
+// into partials[block * 6].
+struct TbAdj {
+  const void* fs[kMaxTimeBlock];
+  double* partials;
+};
+
 // x mod n in [0, n) for any x (n > 0).
 __device__ inline int64_t pmod(int64_t x, int64_t n) {
   x %= n;
   return x < 0 ? x + n : x;
 }
 
-template <typename TS, int R, int K, int WY, bool LO1, bool LO2>
+template <typename TS, int R, int K, int WY, bool LO1, bool LO2, bool PROD = false>
 __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
                                                      TS* __restrict__ out, TbArgs a,
-                                                     Coef<typename St<TS>::M> cf) {
+                                                     Coef<typename St<TS>::M> cf, TbAdj adj) {
   using M = typename St<TS>::M;
   // Edge row handed to the neighbouring wave, double-buffered by the
   // parity of the running level count (one barrier per level suffices).
@@ -829,6 +842,19 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
 #pragma unroll
     for (int r = 0; r < R; ++r) settle(x[r]);
   }
+  // PROD: fv[l] holds the state of level l at the current plane; after
+  // level l it is reloaded with the next plane's (always a valid plane, so
+  // every load is unconditional).
+  M fv[PROD ? K : 1][R];
+  VAcc acc;
+  if constexpr (PROD) {
+#pragma unroll
+    for (int l = 0; l < K; ++l) {
+      const TS* fb = (const TS*)adj.fs[l] + pw * plane;
+#pragma unroll
+      for (int r = 0; r < R; ++r) fv[l][r] = St<TS>::ld((fb + roff[r])[col]);
+    }
+  }
   // One level, in place: row r from level l to l+1 with axis-1 neighbour n1.
   int lvl = 0;
   for (int64_t m = 0; m < cnt; ++m) {
@@ -839,6 +865,7 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
 #pragma unroll
       for (int r = 0; r < R; ++r) nx[r] = St<TS>::ld((base + roff[r])[col]);
     }
+    double ps[3] = {0.0, 0.0, 0.0};  // PROD: this plane's products, all levels
 #pragma unroll
     for (int l = 0; l < K; ++l, ++lvl) {
       // Level l+1 from level l in three phases over all rows -- the three
@@ -851,10 +878,12 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
       constexpr int er = LO1 ? 0 : R - 1;  // the block-edge row of this wave
       if constexpr (WY > 1) edge[buf][w][lane] = x[LO1 ? R - 1 : 0];
       M d0[R], d1[R], d2[R];
+      M nb2[PROD && sizeof(M) < 8 ? R : 1];  // axis-2 neighbours, for FP32 products
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         // The edge lane (63, or 0 with LO2) is a halo column: any value.
         const M n2 = LO2 ? dpp_shift_any<0x138>(x[r]) : dpp_shift_any<0x130>(x[r]);
+        if constexpr (PROD && sizeof(M) < 8) nb2[r] = n2;
         d0[r] = prev[l][r] - x[r];
         d2[r] = n2 - x[r];
         if (r != er) d1[r] = x[LO1 ? r - 1 : r + 1] - x[r];
@@ -866,6 +895,32 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
         if (LO1 ? w > 0 : w + 1 < WY) n1 = e;
       }
       d1[er] = n1 - x[er];
+      if constexpr (PROD) {
+        const bool cells = m >= K && col_out;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const double fd = (double)fv[l][r];
+          double q0, q1, q2;
+          if constexpr (sizeof(M) == 8) {  // the FP64 differences are d0..d2
+            q0 = fd * d0[r];
+            q1 = fd * d1[r];
+            q2 = fd * d2[r];
+          } else {
+            const double xd = (double)x[r];
+            const M v1 = r == er ? n1 : x[LO1 ? (r > 0 ? r - 1 : 0) : (r + 1 < R ? r + 1 : R - 1)];
+            q0 = fd * ((double)prev[l][r] - xd);
+            q1 = fd * ((double)v1 - xd);
+            q2 = fd * ((double)nb2[r] - xd);
+          }
+          const bool ok = cells && row_out[r];
+          ps[0] += ok ? q0 : 0.0;
+          ps[1] += ok ? q1 : 0.0;
+          ps[2] += ok ? q2 : 0.0;
+        }
+        const TS* fb = (const TS*)adj.fs[l] + pn * plane;
+#pragma unroll
+        for (int r = 0; r < R; ++r) fv[l][r] = St<TS>::ld((fb + roff[r])[col]);
+      }
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         d0[r] = cf.c[0] * d0[r];
@@ -882,6 +937,10 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
         x[r] = St<TS>::ld(St<TS>::st(g));  // each level as stored
       }
     }
+    if constexpr (PROD) {
+#pragma unroll
+      for (int j = 0; j < 3; ++j) acc.add(j, ps[j]);
+    }
     if (m >= K && col_out) {
       TS* ob = out + pw * plane;
 #pragma unroll
@@ -894,6 +953,7 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
     }
     pw = pn;
   }
+  if constexpr (PROD) block_reduce_vacc<64 * WY>(acc, adj.partials);
 }
 
 // ------------------------------------------------------------ dispatch ---
@@ -1184,7 +1244,8 @@ hipError_t tb_dispatch(const Grid3& g, const TS* in, TS* out, int K, hipStream_t
     a.chunk = (a.n0 + nch - 1) / nch;                                                       \
     a.ntiles = tiles * ((a.n0 + a.chunk - 1) / a.chunk);                                    \
     if (a.ntiles > 0x7fffffffLL) return hipErrorInvalidConfiguration;                       \
-    hipLaunchKernelGGL(kern, dim3((unsigned)a.ntiles), dim3(64, W), 0, s, in, out, a, cf);  \
+    hipLaunchKernelGGL(kern, dim3((unsigned)a.ntiles), dim3(64, W), 0, s, in, out, a, cf,   \
+                       TbAdj{});                                                            \
     launched = true;                                                                        \
   }
 #define FDB_TB(RR, KK, W)                                                     \
@@ -1199,6 +1260,60 @@ hipError_t tb_dispatch(const Grid3& g, const TS* in, TS* out, int K, hipStream_t
   FDB_TB(5, 8, 8)
 #undef FDB_TB
 #undef FDB_TB_DIR
+  if (!launched) return hipErrorNotSupported;
+  return hipGetLastError();
+}
+
+// Fused transpose pass (tb_kernel with PROD): K adjoint steps with the
+// velocity products, K = 4, 8 waves of 4 rows (FP64) or 8 rows (FP32):
+// prev, the K states and the level's differences then stay within 256
+// VGPRs without spills (FP64 spills from 5 rows on).  *nb_out gets
+// the block count (= partial slots written); with dry = true nothing is
+// launched.  hipErrorNotSupported when the shape has no fused form or
+// needs more than max_blocks partial slots.
+template <typename TS>
+hipError_t tb_adj_dispatch(const Grid3& g, const TS* in, TS* out, const void* const* fs, int K,
+                           double* partials, int64_t max_blocks, int64_t* nb_out, bool dry,
+                           hipStream_t s) {
+  using M = typename St<TS>::M;
+  if (sizeof(TS) < 4 || K > kMaxTimeBlock) return hipErrorNotSupported;
+  constexpr int R = sizeof(TS) == 8 ? 4 : 8, WY = 8;
+  if (!(g.act[0] && g.act[1] && g.act[2]) || WY * R <= K) return hipErrorNotSupported;
+  TbArgs a;
+  a.n0 = g.n[0];
+  a.n1 = g.n[1];
+  a.n2 = g.n[2];
+  a.u0 = -g.up[0];  // the transpose gathers from the downwind side
+  a.tiles2 = (a.n2 + (64 - K) - 1) / (64 - K);
+  a.tiles1 = (a.n1 + (WY * R - K) - 1) / (WY * R - K);
+  const int64_t tiles = a.tiles2 * a.tiles1;
+  const bool lo1 = -g.up[1] < 0, lo2 = -g.up[2] < 0;
+  Coef<M> cf;
+  for (int j = 0; j < 3; ++j) cf.c[j] = (M)g.c[j];
+  TbAdj adj{};
+  for (int l = 0; l < K; ++l) adj.fs[l] = fs ? fs[l] : nullptr;
+  adj.partials = partials;
+  bool launched = false;
+#define FDB_TBA_DIR(KK, L1, L2)                                                              \
+  if (!launched && K == KK && lo1 == L1 && lo2 == L2) {                                      \
+    auto kern = tb_kernel<TS, R, KK, WY, L1, L2, true>;                                      \
+    const int64_t nch =                                                                      \
+        march_chunks(tiles, a.n0, KK, resident_blocks(kern, 64 * WY), 1, 64, true);          \
+    a.chunk = (a.n0 + nch - 1) / nch;                                                        \
+    a.ntiles = tiles * ((a.n0 + a.chunk - 1) / a.chunk);                                     \
+    if (nb_out) *nb_out = a.ntiles;                                                          \
+    if (dry) return hipSuccess;                                                              \
+    if (a.ntiles > max_blocks || a.ntiles > 0x7fffffffLL) return hipErrorNotSupported;       \
+    hipLaunchKernelGGL(kern, dim3((unsigned)a.ntiles), dim3(64, WY), 0, s, in, out, a, cf,   \
+                       adj);                                                                 \
+    launched = true;                                                                         \
+  }
+#define FDB_TBA(KK)                                                  \
+  FDB_TBA_DIR(KK, false, false) FDB_TBA_DIR(KK, false, true)         \
+  FDB_TBA_DIR(KK, true, false) FDB_TBA_DIR(KK, true, true)
+  FDB_TBA(4)
+#undef FDB_TBA
+#undef FDB_TBA_DIR
   if (!launched) return hipErrorNotSupported;
   return hipGetLastError();
 }
@@ -1347,6 +1462,37 @@ hipError_t launch_adjoint_step(int lam_dtype, int f_dtype, const Grid3& g, const
   if (err != hipSuccess || f_t == nullptr) return err;
   hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, partials, nb, 3, acc, 1);
   return hipGetLastError();
+}
+
+hipError_t launch_adjoint_fused(int dtype, const Grid3& g, const void* lam_next, void* lam,
+                                const void* const* fs, int k, double* partials,
+                                int64_t max_blocks, double* acc, hipStream_t s) {
+  if (lam_next == lam || k < 2) return hipErrorNotSupported;
+  int64_t nb = 0;
+  hipError_t err;
+  switch (dtype) {
+    case FDB_F64:
+      err = tb_adj_dispatch<double>(g, (const double*)lam_next, (double*)lam, fs, k, partials,
+                                    max_blocks, &nb, false, s);
+      break;
+    case FDB_F32:
+      err = tb_adj_dispatch<float>(g, (const float*)lam_next, (float*)lam, fs, k, partials,
+                                   max_blocks, &nb, false, s);
+      break;
+    default: return hipErrorNotSupported;
+  }
+  if (err != hipSuccess) return err;
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, partials, nb, 3, acc, 1);
+  return hipGetLastError();
+}
+
+int64_t adjoint_fused_blocks(int dtype, const Grid3& g, int k) {
+  int64_t nb = 0;
+  if (dtype == FDB_F64)
+    (void)tb_adj_dispatch<double>(g, nullptr, nullptr, nullptr, k, nullptr, 0, &nb, true, nullptr);
+  else if (dtype == FDB_F32)
+    (void)tb_adj_dispatch<float>(g, nullptr, nullptr, nullptr, k, nullptr, 0, &nb, true, nullptr);
+  return nb;
 }
 
 int64_t adjoint_blocks(int lam_dtype, int f_dtype, const Grid3& g, const void* lam_next,

@@ -205,6 +205,39 @@ const void* top_buffer(const fdb_tape* t) {
 // The slot the next STORE fills.
 void* next_slot(fdb_tape* t) { return t->slots[t->stack_steps.size() - 1]; }
 
+// Buffer of stack entry idx (entry 0 is the initial state).
+const void* stack_buffer(const fdb_tape* t, size_t idx) {
+  return idx == 0 ? t->base : t->slots[idx - 1];
+}
+
+// Steps in the fused transpose pass.
+constexpr int kFusedAdjointSteps = 4;
+
+// Length of the fused adjoint run starting at acts[i] = ADJOINT t: the
+// schedule continues with (POP, RESTORE, ADJOINT t-1), (POP, RESTORE,
+// ADJOINT t-2), ... and states t-1, t-2, ... are the stored states just
+// below the stack top, so one pass can read them all.  Returns
+// kFusedAdjointSteps when that many steps qualify (and the storage types
+// have a fused form), else 1.
+int fused_adjoint_run(const fdb_tape* t, size_t i) {
+  constexpr int k = kFusedAdjointSteps;
+  if (t->time_block < k || t->f_dtype != t->lam_dtype ||
+      (t->f_dtype != FDB_F64 && t->f_dtype != FDB_F32))
+    return 1;
+  const auto& acts = t->plan.acts;
+  const int step = acts[i].second;
+  const size_t n = t->stack_steps.size();
+  if (n < (size_t)k || t->stack_steps[n - 1] != step) return 1;
+  for (int q = 1; q < k; ++q) {
+    const size_t j = i + 3 * (size_t)q;
+    if (j >= acts.size() || acts[j - 2].first != FDB_CK_POP ||
+        acts[j - 1].first != FDB_CK_RESTORE || acts[j].first != FDB_CK_ADJOINT ||
+        acts[j].second != step - q || t->stack_steps[n - 1 - q] != step - q)
+      return 1;
+  }
+  return k;
+}
+
 // Executes plan actions from index i; in forward mode stops at the first
 // RESTORE or ADJOINT and returns its index through *stop.
 int run_plan(fdb_tape* t, size_t i, bool forward_only, hipStream_t s, const void** lam_next,
@@ -253,9 +286,36 @@ int run_plan(fdb_tape* t, size_t i, bool forward_only, hipStream_t s, const void
         }
         if (t->wstep != a.second)
           return tfail(FDB_ERR_INVALID, "tape_backward: schedule out of step (internal)");
-        ++*done;
-        // The last adjoint step writes into lam0.
+        // lam ping-pongs between lam0 and lam_tmp, aiming for the last step
+        // to land in lam0 (fdb_tape_backward copies it there otherwise).
+        const int k = fused_adjoint_run(t, i);
+        *done += k;
         void* dst = ((t->nsteps - *done) % 2 == 0) ? lam0 : t->lam_tmp;
+        if (dst == *lam_next) dst = (dst == lam0) ? t->lam_tmp : lam0;
+        if (k > 1) {
+          // One pass over steps t, t-1, ..., t-k+1 (states from the stack),
+          // standing for the k ADJOINTs and the POP/RESTOREs between them.
+          const void* fs[kFusedAdjointSteps];
+          const size_t n = t->stack_steps.size();
+          fs[0] = t->w;
+          for (int q = 1; q < k; ++q) fs[q] = stack_buffer(t, n - 1 - q);
+          const hipError_t e = fdb::launch_adjoint_fused(
+              t->f_dtype, t->g, *lam_next, dst, fs, k, t->partials,
+              (int64_t)(t->partials_bytes / (6 * sizeof(double))), t->acc, s);
+          if (e == hipSuccess) {
+            for (int q = 1; q < k; ++q) t->stack_steps.pop_back();
+            t->w = fs[k - 1];
+            t->wstep = a.second - (k - 1);
+            i += 3 * (size_t)(k - 1);  // the last ADJOINT of the run
+            *lam_next = dst;
+            t->adj_steps += k;
+            break;
+          }
+          if (e != hipErrorNotSupported) TAPE_HIP(e);
+          *done -= k - 1;  // not fused after all: this step alone
+          dst = ((t->nsteps - *done) % 2 == 0) ? lam0 : t->lam_tmp;
+          if (dst == *lam_next) dst = (dst == lam0) ? t->lam_tmp : lam0;
+        }
         TAPE_HIP(fdb::launch_adjoint_step(t->lam_dtype, t->f_dtype, t->g, *lam_next, dst, t->w,
                                           t->partials, t->acc, s));
         *lam_next = dst;
@@ -378,6 +438,8 @@ int fdb_tape_backward(fdb_tape_t* t, const void* lam_T, void* lam0, double* grad
   for (auto& p : pairs)
     nb = std::max(nb, fdb::adjoint_blocks(t->lam_dtype, t->f_dtype, t->g, p[0],
                                           const_cast<void*>(p[1]), t->base));
+  if (t->f_dtype == t->lam_dtype && t->time_block >= kFusedAdjointSteps)
+    nb = std::max(nb, fdb::adjoint_fused_blocks(t->f_dtype, t->g, kFusedAdjointSteps));
   const size_t pb = (size_t)nb * 6 * sizeof(double);
   if (pb > t->partials_bytes) {
     if (t->partials) TAPE_HIP(hipFree(t->partials));
@@ -391,6 +453,8 @@ int fdb_tape_backward(fdb_tape_t* t, const void* lam_T, void* lam0, double* grad
   int done = 0;
   if (int rc = run_plan(t, t->resume, false, s, &lam_next, lam0, &done, nullptr)) return rc;
   if (done != t->nsteps) return tfail(FDB_ERR_INVALID, "tape_backward: incomplete (internal)");
+  if (lam_next != lam0)
+    TAPE_HIP(hipMemcpyAsync(lam0, lam_next, t->lbytes, hipMemcpyDeviceToDevice, s));
   double acc[6];
   TAPE_HIP(hipMemcpyAsync(acc, t->acc, sizeof(acc), hipMemcpyDeviceToHost, s));
   TAPE_HIP(hipStreamSynchronize(s));
@@ -402,6 +466,13 @@ int fdb_tape_backward(fdb_tape_t* t, const void* lam_T, void* lam0, double* grad
   // The sweep consumed the stored states: another backward needs another
   // forward.
   t->forward_done = false;
+  return FDB_OK;
+}
+
+int fdb_tape_set_time_block(fdb_tape_t* t, int k) {
+  if (!t || k < 1 || k > fdb::kMaxTimeBlock)
+    return tfail(FDB_ERR_INVALID, "tape_set_time_block: k must be 1..8");
+  t->time_block = k;
   return FDB_OK;
 }
 

@@ -170,6 +170,13 @@ int fdb_tape_create(int ndims, const uint64_t* num_cells, const double* velocity
 int fdb_tape_forward(fdb_tape_t* t, const void* f0, void* fT, void* stream);
 int fdb_tape_backward(fdb_tape_t* t, const void* lam_T, void* lam0, double* grad_v,
                       void* stream);
+/* Steps per launch, 1..8 (default 8): runs of forward steps between stored
+ * states go through fused multi-step passes, and runs of adjoint steps
+ * whose states are all stored (FDB_F64 or FDB_F32, f and lam of one type,
+ * 3-D) through a fused 4-step transpose pass.  lam_0 is bit-identical for
+ * every k; dL/dv sums the same products in another order (equal to
+ * rounding).  Extension (no counterpart in the reference). */
+int fdb_tape_set_time_block(fdb_tape_t* t, int k);
 /* Checkpoint slots used, forward steps and adjoint steps executed so far. */
 int fdb_tape_stats(const fdb_tape_t* t, int* snaps, int64_t* forward_steps,
                    int64_t* adjoint_steps);

@@ -182,14 +182,59 @@ def test_gradient_matches_oracle(oracle):
 
 @pytest.mark.parametrize("snaps", [0, 1, 3])
 def test_checkpointing_is_exact(snaps):
-    """Few snapshots (recomputation) gives the same bits as storing all."""
+    """Few snapshots (recomputation) gives the same dL/df0 bits as storing
+    all; dL/dv sums the same products, in another order where storing all
+    lets the adjoint fuse 4 steps per pass."""
     shape, v, dt, T, f0, target = _setup(shape=(12, 16, 64), T=23, seed=5)
     full = _tape_grad(f0, target, v, dt, T)
     fbytes = int(np.prod(shape)) * 8
     # fixed working set: initial state, two working buffers, one adjoint buffer
     part = _tape_grad(f0, target, v, dt, T, max_bytes=(4 + snaps) * fbytes)
     assert np.array_equal(bits(full[1]), bits(part[1]))
-    assert np.array_equal(bits(full[2]), bits(part[2]))
+    assert np.allclose(full[2], part[2], rtol=1e-13, atol=0)
+
+
+def _fused_vs_single(shape, v, dt, T, fd, tdt, seed):
+    """The tape at 8 steps per launch (fused 4-step adjoint passes over the
+    stored states) against 1 step per launch."""
+    from fidibench_amd.autograd import Tape
+    g = torch.Generator().manual_seed(seed)
+    f0 = torch.rand(shape, generator=g, dtype=torch.float64).to(tdt).cuda()
+    lamT = torch.randn(shape, generator=g, dtype=torch.float64).to(tdt).cuda()
+    out = []
+    for k in (8, 1):
+        tape = Tape(shape, v, (1, 1, 1), dt, T, fd, fd, 1 << 31)
+        tape.set_time_block(k)
+        fT = torch.empty_like(f0)
+        tape.forward(f0, fT)
+        lam0 = torch.empty_like(lamT)
+        gv = tape.backward(lamT, lam0)
+        st = tape.stats()
+        tape.close()
+        assert st["adjoint_steps"] == T and st["snapshots"] == T - 1
+        out.append((fT.cpu(), lam0.cpu(), np.array(gv)))
+    (f8, l8, g8), (f1, l1, g1) = out
+    assert torch.equal(f8, f1)
+    assert torch.equal(l8.view(torch.int64 if tdt == torch.float64 else torch.int32),
+                       l1.view(torch.int64 if tdt == torch.float64 else torch.int32))
+    assert np.allclose(g8, g1, rtol=1e-12, atol=0), (g8, g1)
+
+
+# every sign pairing of the three axes (the fused pass's four direction
+# variants and both march directions), tiles cut by the grid edges, step
+# counts that leave 1-3 single steps after the 4-step passes
+@pytest.mark.parametrize("v", [(0.45, 0.3, 0.2), (-0.45, -0.3, -0.2), (0.4, -0.35, 0.25),
+                               (-0.3, 0.45, -0.2)])
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_fused_adjoint_matches_single_steps(v, dtype):
+    from fidibench_amd import _lib as L
+    fd, tdt = (L.F64, torch.float64) if dtype == "f64" else (L.F32, torch.float32)
+    _fused_vs_single((20, 70, 130), v, 0.003, 11, fd, tdt, seed=len(dtype) + int(10 * v[0]))
+
+
+def test_fused_adjoint_long_run():
+    from fidibench_amd import _lib as L
+    _fused_vs_single((33, 45, 200), (0.6, -0.2, 0.3), 0.002, 26, L.F64, torch.float64, seed=7)
 
 
 def test_checkpointing_exact_with_stochastic_rounding():
