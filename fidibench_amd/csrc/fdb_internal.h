@@ -56,10 +56,12 @@ hipError_t launch_advect_pass(int dtype, const Grid3& g, void* const bufs[2], in
                               uint64_t seed = 0);
 
 // One launch of k >= 2 fused steps from src into dst (distinct buffers);
-// src is not written.  hipErrorNotSupported when no single launch does k
-// steps for this storage type and shape (the caller then steps one by one).
+// src is not written.  With levels != nullptr the states after steps
+// 1..k-1 are also written, to levels[0..k-2] (FP64, k = 4 or 8).
+// hipErrorNotSupported when no single launch does this for the storage
+// type and shape (the caller then steps one by one).
 hipError_t launch_fused_pass(int dtype, const Grid3& g, const void* src, void* dst, int k,
-                             hipStream_t s);
+                             hipStream_t s, void* const* levels = nullptr);
 
 // The storage layout of a dtype (FDB_BF16_SR is stored as FDB_BF16).
 inline int storage_dtype(int dtype) { return dtype == 3 ? 2 : dtype; }

@@ -765,16 +765,23 @@ struct TbArgs {
   int64_t tiles2, tiles1, ntiles;
 };
 
-// PROD = true: the fused transpose (adjoint) pass.  The kernel is called
-// with the adjoint's directions (-u), so level l+1 = A^T (level l), and
-// fs[l] is the forward state of that adjoint step (level l of the pass
-// consumes f_{t-l} when the pass starts from lam_{t+1}).  At every output
-// cell it also sums f[i] * (lam[i + dir_j e_j] - lam[i]) for j = 0, 1, 2 --
-// the one-step adjoint's velocity products -- in FP64 (differences taken
-// in FP64 as there), Neumaier-compensated per thread and reduced per block
-// into partials[block * 6].
-struct TbAdj {
+// Modes of the temporally blocked kernel.
+//   kTbPlain     K forward steps, level K stored to out.
+//   kTbStoreAll  also levels 1..K-1 stored, to aux.lv[0..K-2] (a forward
+//                that keeps every state).
+//   kTbAdjoint   the fused transpose (adjoint) pass.  The kernel is called
+//                with the adjoint's directions (-u), so level l+1 =
+//                A^T (level l), and aux.fs[l] is the forward state of that
+//                adjoint step (level l of the pass consumes f_{t-l} when the
+//                pass starts from lam_{t+1}).  At every output cell it also
+//                sums f[i] * (lam[i + dir_j e_j] - lam[i]) for j = 0, 1, 2 --
+//                the one-step adjoint's velocity products -- in FP64
+//                (differences taken in FP64 as there), Neumaier-compensated
+//                per thread and reduced per block into aux.partials[block*6].
+constexpr int kTbPlain = 0, kTbStoreAll = 1, kTbAdjoint = 2;
+struct TbAux {
   const void* fs[kMaxTimeBlock];
+  void* lv[kMaxTimeBlock];
   double* partials;
 };
 
@@ -784,11 +791,12 @@ __device__ inline int64_t pmod(int64_t x, int64_t n) {
   return x < 0 ? x + n : x;
 }
 
-template <typename TS, int R, int K, int WY, bool LO1, bool LO2, bool PROD = false>
+template <typename TS, int R, int K, int WY, bool LO1, bool LO2, int MODE = kTbPlain>
 __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
                                                      TS* __restrict__ out, TbArgs a,
-                                                     Coef<typename St<TS>::M> cf, TbAdj adj) {
+                                                     Coef<typename St<TS>::M> cf, TbAux adj) {
   using M = typename St<TS>::M;
+  constexpr bool PROD = MODE == kTbAdjoint, ALL = MODE == kTbStoreAll;
   // Edge row handed to the neighbouring wave, double-buffered by the
   // parity of the running level count (one barrier per level suffices).
   __shared__ M edge[2][WY][64];
@@ -935,6 +943,14 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
         g = g - d2[r];
         prev[l][r] = x[r];
         x[r] = St<TS>::ld(St<TS>::st(g));  // each level as stored
+      }
+      if constexpr (ALL) {
+        if (l + 1 < K && m >= K && col_out) {  // level l+1 (level K below)
+          TS* ob = (TS*)adj.lv[l] + pw * plane;
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+            if (row_out[r]) __builtin_nontemporal_store(St<TS>::st(x[r]), (ob + roff[r]) + col);
+        }
       }
     }
     if constexpr (PROD) {
@@ -1211,8 +1227,12 @@ TbCfg tb_cfg(int bytes, int K) {
   return c;
 }
 
+// levels != nullptr: also store levels 1..K-1 to levels[0..K-2] (K = 4
+// or 8, FP64 only: with one 4-byte cell per lane the FP32 kernel writes its
+// 8 levels at ~2 TB/s, slower than one-step launches storing 16 B per lane).
 template <typename TS>
-hipError_t tb_dispatch(const Grid3& g, const TS* in, TS* out, int K, hipStream_t s) {
+hipError_t tb_dispatch(const Grid3& g, const TS* in, TS* out, int K, hipStream_t s,
+                       void* const* levels = nullptr) {
   using M = typename St<TS>::M;
   // 2-byte storage: the one-step kernel moves 16 bytes per lane and is
   // faster than any fused pass with one cell per lane (measured).
@@ -1231,10 +1251,15 @@ hipError_t tb_dispatch(const Grid3& g, const TS* in, TS* out, int K, hipStream_t
   const bool lo1 = g.up[1] < 0, lo2 = g.up[2] < 0;
   Coef<M> cf;
   for (int j = 0; j < 3; ++j) cf.c[j] = (M)g.c[j];
+  TbAux aux{};
+  const bool all = levels != nullptr;
+  if (all && sizeof(TS) != 8) return hipErrorNotSupported;
+  for (int l = 0; all && l + 1 < K; ++l) aux.lv[l] = levels[l];
   bool launched = false;
-#define FDB_TB_DIR(RR, KK, W, L1, L2)                                                       \
-  if (!launched && R == RR && K == KK && WY == W && lo1 == L1 && lo2 == L2) {               \
-    auto kern = tb_kernel<TS, RR, KK, W, L1, L2>;                                           \
+#define FDB_TB_DIR(RR, KK, W, L1, L2, MD)                                                   \
+  if (!launched && R == RR && K == KK && WY == W && lo1 == L1 && lo2 == L2 &&               \
+      all == (MD == kTbStoreAll)) {                                                         \
+    auto kern = tb_kernel<TS, RR, KK, W, L1, L2, MD>;                                       \
     int64_t nch;                                                                            \
     if (cfg.blocks > 0) {                                                                   \
       nch = std::max<int64_t>(1, (cfg.blocks + tiles / 2) / tiles);                         \
@@ -1245,19 +1270,23 @@ hipError_t tb_dispatch(const Grid3& g, const TS* in, TS* out, int K, hipStream_t
     a.ntiles = tiles * ((a.n0 + a.chunk - 1) / a.chunk);                                    \
     if (a.ntiles > 0x7fffffffLL) return hipErrorInvalidConfiguration;                       \
     hipLaunchKernelGGL(kern, dim3((unsigned)a.ntiles), dim3(64, W), 0, s, in, out, a, cf,   \
-                       TbAdj{});                                                            \
+                       aux);                                                                \
     launched = true;                                                                        \
   }
-#define FDB_TB(RR, KK, W)                                                     \
-  FDB_TB_DIR(RR, KK, W, false, false) FDB_TB_DIR(RR, KK, W, false, true)      \
-  FDB_TB_DIR(RR, KK, W, true, false) FDB_TB_DIR(RR, KK, W, true, true)
-  FDB_TB(9, 2, 4)
-  FDB_TB(9, 3, 4)
-  FDB_TB(9, 4, 4)
-  FDB_TB(9, 6, 8)
-  FDB_TB(9, 7, 8)
-  FDB_TB(9, 8, 8)
-  FDB_TB(5, 8, 8)
+#define FDB_TB(RR, KK, W, MD)                                                          \
+  FDB_TB_DIR(RR, KK, W, false, false, MD) FDB_TB_DIR(RR, KK, W, false, true, MD)       \
+  FDB_TB_DIR(RR, KK, W, true, false, MD) FDB_TB_DIR(RR, KK, W, true, true, MD)
+  FDB_TB(9, 2, 4, kTbPlain)
+  FDB_TB(9, 3, 4, kTbPlain)
+  FDB_TB(9, 4, 4, kTbPlain)
+  FDB_TB(9, 6, 8, kTbPlain)
+  FDB_TB(9, 7, 8, kTbPlain)
+  FDB_TB(9, 8, 8, kTbPlain)
+  FDB_TB(5, 8, 8, kTbPlain)
+  if constexpr (sizeof(TS) == 8) {
+    FDB_TB(9, 4, 4, kTbStoreAll)
+    FDB_TB(9, 8, 8, kTbStoreAll)
+  }
 #undef FDB_TB
 #undef FDB_TB_DIR
   if (!launched) return hipErrorNotSupported;
@@ -1290,13 +1319,13 @@ hipError_t tb_adj_dispatch(const Grid3& g, const TS* in, TS* out, const void* co
   const bool lo1 = -g.up[1] < 0, lo2 = -g.up[2] < 0;
   Coef<M> cf;
   for (int j = 0; j < 3; ++j) cf.c[j] = (M)g.c[j];
-  TbAdj adj{};
+  TbAux adj{};
   for (int l = 0; l < K; ++l) adj.fs[l] = fs ? fs[l] : nullptr;
   adj.partials = partials;
   bool launched = false;
 #define FDB_TBA_DIR(KK, L1, L2)                                                              \
   if (!launched && K == KK && lo1 == L1 && lo2 == L2) {                                      \
-    auto kern = tb_kernel<TS, R, KK, WY, L1, L2, true>;                                      \
+    auto kern = tb_kernel<TS, R, KK, WY, L1, L2, kTbAdjoint>;                                \
     const int64_t nch =                                                                      \
         march_chunks(tiles, a.n0, KK, resident_blocks(kern, 64 * WY), 1, 64, true);          \
     a.chunk = (a.n0 + nch - 1) / nch;                                                        \
@@ -1391,11 +1420,11 @@ bool make_grid(int ndims, const uint64_t* n, const double* v, const double* len,
 }
 
 hipError_t launch_fused_pass(int dtype, const Grid3& g, const void* src, void* dst, int k,
-                             hipStream_t s) {
+                             hipStream_t s, void* const* levels) {
   if (k < 2 || src == dst) return hipErrorNotSupported;
   switch (dtype) {
-    case FDB_F64: return tb_dispatch<double>(g, (const double*)src, (double*)dst, k, s);
-    case FDB_F32: return tb_dispatch<float>(g, (const float*)src, (float*)dst, k, s);
+    case FDB_F64: return tb_dispatch<double>(g, (const double*)src, (double*)dst, k, s, levels);
+    case FDB_F32: return tb_dispatch<float>(g, (const float*)src, (float*)dst, k, s, levels);
     default: return hipErrorNotSupported;
   }
 }

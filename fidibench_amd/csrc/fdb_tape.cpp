@@ -248,6 +248,33 @@ int run_plan(fdb_tape* t, size_t i, bool forward_only, hipStream_t s, const void
     switch (a.first) {
       case FDB_CK_ADVANCE: {
         const bool store_next = i + 1 < acts.size() && acts[i + 1].first == FDB_CK_STORE;
+        // A run of (ADVANCE 1, STORE) pairs -- the schedule keeping every
+        // state -- goes through fused passes that write each intermediate
+        // state into its snapshot slot.
+        if (store_next && a.second == 1) {
+          size_t run = 0;
+          while (i + 2 * run + 1 < acts.size() && acts[i + 2 * run].first == FDB_CK_ADVANCE &&
+                 acts[i + 2 * run].second == 1 && acts[i + 2 * run + 1].first == FDB_CK_STORE)
+            ++run;
+          const int k = (run >= 8 && t->time_block >= 8) ? 8
+                        : (run >= 4 && t->time_block >= 4) ? 4 : 0;
+          if (k > 0) {
+            const size_t n = t->stack_steps.size();
+            void* lv[fdb::kMaxTimeBlock];
+            for (int q = 0; q < k; ++q) lv[q] = t->slots[n - 1 + q];
+            const hipError_t e =
+                fdb::launch_fused_pass(t->f_dtype, t->g, t->w, lv[k - 1], k, s, lv);
+            if (e == hipSuccess) {
+              for (int q = 0; q < k; ++q) t->stack_steps.push_back(t->wstep + q + 1);
+              t->w = lv[k - 1];
+              t->wstep += k;
+              t->fwd_steps += k;
+              i += 2 * (size_t)k - 1;  // the last STORE of the run
+              break;
+            }
+            if (e != hipErrorNotSupported) TAPE_HIP(e);
+          }
+        }
         if (int rc = advance(t, a.second, store_next ? next_slot(t) : nullptr, s)) return rc;
         break;
       }

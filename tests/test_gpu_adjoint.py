@@ -237,6 +237,16 @@ def test_fused_adjoint_long_run():
     _fused_vs_single((33, 45, 200), (0.6, -0.2, 0.3), 0.002, 26, L.F64, torch.float64, seed=7)
 
 
+@pytest.mark.parametrize("T", [5, 6, 9])
+def test_fused_store_all_forward_short_runs(T):
+    """Runs of 4..8 stored states: 4-level and 8-level store-all forward
+    passes plus single steps, each state bit-identical (the fused adjoint
+    reads them back)."""
+    from fidibench_amd import _lib as L
+    _fused_vs_single((24, 40, 100), (-0.5, 0.25, -0.35), 0.003, T, L.F64, torch.float64,
+                     seed=T)
+
+
 def test_checkpointing_exact_with_stochastic_rounding():
     """bf16 storage with stochastic rounding: recomputed states use the same
     random bits (keyed by the absolute step), so few snapshots give the
