@@ -66,8 +66,8 @@ hipError_t launch_fused_pass(int dtype, const Grid3& g, const void* src, void* d
 // The storage layout of a dtype (FDB_BF16_SR is stored as FDB_BF16).
 inline int storage_dtype(int dtype) { return dtype == 3 ? 2 : dtype; }
 
-// K transpose steps in one pass (FP64 or FP32 storage, lam and f_t of the
-// same type, all three axes present): lam = (A^T)^K lam_next, where fs[l]
+// K transpose steps in one pass (lam/f_t storage FP64/FP64, FP32/FP32 or
+// FP32/bf16; all three axes present): lam = (A^T)^K lam_next, where fs[l]
 // is the forward state of the l-th step (f_{t-l} for a pass starting from
 // lam_{t+1}); the velocity products of all K steps are added into acc
 // (device, 3 x {sum, comp}), as K launch_adjoint_step calls would add
@@ -75,10 +75,10 @@ inline int storage_dtype(int dtype) { return dtype == 3 ? 2 : dtype; }
 // order).  partials needs adjoint_fused_blocks(...) slots of 6 doubles;
 // hipErrorNotSupported when the shape has no fused form or needs more than
 // max_blocks slots.
-hipError_t launch_adjoint_fused(int dtype, const Grid3& g, const void* lam_next, void* lam,
-                                const void* const* fs, int k, double* partials,
+hipError_t launch_adjoint_fused(int lam_dtype, int f_dtype, const Grid3& g, const void* lam_next,
+                                void* lam, const void* const* fs, int k, double* partials,
                                 int64_t max_blocks, double* acc, hipStream_t s);
-int64_t adjoint_fused_blocks(int dtype, const Grid3& g, int k);
+int64_t adjoint_fused_blocks(int lam_dtype, int f_dtype, const Grid3& g, int k);
 
 // Launch sizing shared by the marching kernels.
 int device_cus();

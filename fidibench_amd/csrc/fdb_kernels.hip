@@ -791,7 +791,8 @@ __device__ inline int64_t pmod(int64_t x, int64_t n) {
   return x < 0 ? x + n : x;
 }
 
-template <typename TS, int R, int K, int WY, bool LO1, bool LO2, int MODE = kTbPlain>
+template <typename TS, int R, int K, int WY, bool LO1, bool LO2, int MODE = kTbPlain,
+          typename TF = TS>
 __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
                                                      TS* __restrict__ out, TbArgs a,
                                                      Coef<typename St<TS>::M> cf, TbAux adj) {
@@ -858,9 +859,9 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
   if constexpr (PROD) {
 #pragma unroll
     for (int l = 0; l < K; ++l) {
-      const TS* fb = (const TS*)adj.fs[l] + pw * plane;
+      const TF* fb = (const TF*)adj.fs[l] + pw * plane;
 #pragma unroll
-      for (int r = 0; r < R; ++r) fv[l][r] = St<TS>::ld((fb + roff[r])[col]);
+      for (int r = 0; r < R; ++r) fv[l][r] = (M)St<TF>::ld((fb + roff[r])[col]);
     }
   }
   // One level, in place: row r from level l to l+1 with axis-1 neighbour n1.
@@ -925,9 +926,9 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
           ps[1] += ok ? q1 : 0.0;
           ps[2] += ok ? q2 : 0.0;
         }
-        const TS* fb = (const TS*)adj.fs[l] + pn * plane;
+        const TF* fb = (const TF*)adj.fs[l] + pn * plane;
 #pragma unroll
-        for (int r = 0; r < R; ++r) fv[l][r] = St<TS>::ld((fb + roff[r])[col]);
+        for (int r = 0; r < R; ++r) fv[l][r] = (M)St<TF>::ld((fb + roff[r])[col]);
       }
 #pragma unroll
       for (int r = 0; r < R; ++r) {
@@ -1300,7 +1301,7 @@ hipError_t tb_dispatch(const Grid3& g, const TS* in, TS* out, int K, hipStream_t
 // the block count (= partial slots written); with dry = true nothing is
 // launched.  hipErrorNotSupported when the shape has no fused form or
 // needs more than max_blocks partial slots.
-template <typename TS>
+template <typename TS, typename TF = TS>
 hipError_t tb_adj_dispatch(const Grid3& g, const TS* in, TS* out, const void* const* fs, int K,
                            double* partials, int64_t max_blocks, int64_t* nb_out, bool dry,
                            hipStream_t s) {
@@ -1325,7 +1326,7 @@ hipError_t tb_adj_dispatch(const Grid3& g, const TS* in, TS* out, const void* co
   bool launched = false;
 #define FDB_TBA_DIR(KK, L1, L2)                                                              \
   if (!launched && K == KK && lo1 == L1 && lo2 == L2) {                                      \
-    auto kern = tb_kernel<TS, R, KK, WY, L1, L2, kTbAdjoint>;                                \
+    auto kern = tb_kernel<TS, R, KK, WY, L1, L2, kTbAdjoint, TF>;                            \
     const int64_t nch =                                                                      \
         march_chunks(tiles, a.n0, KK, resident_blocks(kern, 64 * WY), 1, 64, true);          \
     a.chunk = (a.n0 + nch - 1) / nch;                                                        \
@@ -1493,34 +1494,44 @@ hipError_t launch_adjoint_step(int lam_dtype, int f_dtype, const Grid3& g, const
   return hipGetLastError();
 }
 
-hipError_t launch_adjoint_fused(int dtype, const Grid3& g, const void* lam_next, void* lam,
-                                const void* const* fs, int k, double* partials,
+namespace {
+
+// The (lam, f_t) storage pairings with a fused transpose pass: FP64/FP64,
+// FP32/FP32 and FP32 lam over bf16 states (the mixed-precision gradient).
+hipError_t adj_fused_typed(int lam_dtype, int f_dtype, const Grid3& g, const void* lam_next,
+                           void* lam, const void* const* fs, int k, double* partials,
+                           int64_t max_blocks, int64_t* nb, bool dry, hipStream_t s) {
+  f_dtype = storage_dtype(f_dtype);
+  if (lam_dtype == FDB_F64 && f_dtype == FDB_F64)
+    return tb_adj_dispatch<double>(g, (const double*)lam_next, (double*)lam, fs, k, partials,
+                                   max_blocks, nb, dry, s);
+  if (lam_dtype == FDB_F32 && f_dtype == FDB_F32)
+    return tb_adj_dispatch<float>(g, (const float*)lam_next, (float*)lam, fs, k, partials,
+                                  max_blocks, nb, dry, s);
+  if (lam_dtype == FDB_F32 && f_dtype == FDB_BF16)
+    return tb_adj_dispatch<float, uint16_t>(g, (const float*)lam_next, (float*)lam, fs, k,
+                                            partials, max_blocks, nb, dry, s);
+  return hipErrorNotSupported;
+}
+
+}  // namespace
+
+hipError_t launch_adjoint_fused(int lam_dtype, int f_dtype, const Grid3& g, const void* lam_next,
+                                void* lam, const void* const* fs, int k, double* partials,
                                 int64_t max_blocks, double* acc, hipStream_t s) {
   if (lam_next == lam || k < 2) return hipErrorNotSupported;
   int64_t nb = 0;
-  hipError_t err;
-  switch (dtype) {
-    case FDB_F64:
-      err = tb_adj_dispatch<double>(g, (const double*)lam_next, (double*)lam, fs, k, partials,
-                                    max_blocks, &nb, false, s);
-      break;
-    case FDB_F32:
-      err = tb_adj_dispatch<float>(g, (const float*)lam_next, (float*)lam, fs, k, partials,
-                                   max_blocks, &nb, false, s);
-      break;
-    default: return hipErrorNotSupported;
-  }
+  const hipError_t err = adj_fused_typed(lam_dtype, f_dtype, g, lam_next, lam, fs, k, partials,
+                                         max_blocks, &nb, false, s);
   if (err != hipSuccess) return err;
   hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, partials, nb, 3, acc, 1);
   return hipGetLastError();
 }
 
-int64_t adjoint_fused_blocks(int dtype, const Grid3& g, int k) {
+int64_t adjoint_fused_blocks(int lam_dtype, int f_dtype, const Grid3& g, int k) {
   int64_t nb = 0;
-  if (dtype == FDB_F64)
-    (void)tb_adj_dispatch<double>(g, nullptr, nullptr, nullptr, k, nullptr, 0, &nb, true, nullptr);
-  else if (dtype == FDB_F32)
-    (void)tb_adj_dispatch<float>(g, nullptr, nullptr, nullptr, k, nullptr, 0, &nb, true, nullptr);
+  (void)adj_fused_typed(lam_dtype, f_dtype, g, nullptr, nullptr, nullptr, k, nullptr, 0, &nb,
+                        true, nullptr);
   return nb;
 }
 

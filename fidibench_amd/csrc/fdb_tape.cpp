@@ -122,6 +122,7 @@ struct fdb_tape {
   size_t partials_bytes = 0;
   double* acc = nullptr;
   bool forward_done = false;
+  bool fuse_adj = false;  // this backward may use fused transpose passes
   int64_t fwd_steps = 0, adj_steps = 0;
 };
 
@@ -221,9 +222,7 @@ constexpr int kFusedAdjointSteps = 4;
 // have a fused form), else 1.
 int fused_adjoint_run(const fdb_tape* t, size_t i) {
   constexpr int k = kFusedAdjointSteps;
-  if (t->time_block < k || t->f_dtype != t->lam_dtype ||
-      (t->f_dtype != FDB_F64 && t->f_dtype != FDB_F32))
-    return 1;
+  if (!t->fuse_adj) return 1;
   const auto& acts = t->plan.acts;
   const int step = acts[i].second;
   const size_t n = t->stack_steps.size();
@@ -327,7 +326,7 @@ int run_plan(fdb_tape* t, size_t i, bool forward_only, hipStream_t s, const void
           fs[0] = t->w;
           for (int q = 1; q < k; ++q) fs[q] = stack_buffer(t, n - 1 - q);
           const hipError_t e = fdb::launch_adjoint_fused(
-              t->f_dtype, t->g, *lam_next, dst, fs, k, t->partials,
+              t->lam_dtype, t->f_dtype, t->g, *lam_next, dst, fs, k, t->partials,
               (int64_t)(t->partials_bytes / (6 * sizeof(double))), t->acc, s);
           if (e == hipSuccess) {
             for (int q = 1; q < k; ++q) t->stack_steps.pop_back();
@@ -465,8 +464,13 @@ int fdb_tape_backward(fdb_tape_t* t, const void* lam_T, void* lam0, double* grad
   for (auto& p : pairs)
     nb = std::max(nb, fdb::adjoint_blocks(t->lam_dtype, t->f_dtype, t->g, p[0],
                                           const_cast<void*>(p[1]), t->base));
-  if (t->f_dtype == t->lam_dtype && t->time_block >= kFusedAdjointSteps)
-    nb = std::max(nb, fdb::adjoint_fused_blocks(t->f_dtype, t->g, kFusedAdjointSteps));
+  t->fuse_adj = false;
+  if (t->time_block >= kFusedAdjointSteps) {
+    const int64_t fb =
+        fdb::adjoint_fused_blocks(t->lam_dtype, t->f_dtype, t->g, kFusedAdjointSteps);
+    t->fuse_adj = fb > 0;
+    nb = std::max(nb, fb);
+  }
   const size_t pb = (size_t)nb * 6 * sizeof(double);
   if (pb > t->partials_bytes) {
     if (t->partials) TAPE_HIP(hipFree(t->partials));

@@ -172,7 +172,7 @@ int fdb_tape_backward(fdb_tape_t* t, const void* lam_T, void* lam0, double* grad
                       void* stream);
 /* Steps per launch, 1..8 (default 8): runs of forward steps between stored
  * states go through fused multi-step passes, and runs of adjoint steps
- * whose states are all stored (FDB_F64 or FDB_F32, f and lam of one type,
+ * whose states are all stored (lam/f storage FP64/FP64, FP32/FP32 or FP32/bf16,
  * 3-D) through a fused 4-step transpose pass.  lam_0 is bit-identical for
  * every k; dL/dv sums the same products in another order (equal to
  * rounding).  Extension (no counterpart in the reference). */
