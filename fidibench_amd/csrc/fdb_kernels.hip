@@ -853,15 +853,28 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
   }
   // PROD: fv[l] holds the state of level l at the current plane; after
   // level l it is reloaded with the next plane's (always a valid plane, so
-  // every load is unconditional).
+  // every load is unconditional).  The states are needed at output cells
+  // only: halo lanes and rows load the nearest output cell of the tile
+  // instead (a line the tile loads anyway), so the K states cost no halo
+  // traffic.
   M fv[PROD ? K : 1][R];
   VAcc acc;
+  uint32_t colf = col;
+  int64_t rofff[PROD ? R : 1];
   if constexpr (PROD) {
+    const int64_t c_hi = (o2 + 63 - K < a.n2 - 1) ? o2 + 63 - K : a.n2 - 1;
+    colf = (uint32_t)(cl < o2 ? o2 : (cl > c_hi ? c_hi : cl));
+    const int64_t r_hi = (o1 + WY * R - K - 1 < a.n1 - 1) ? o1 + WY * R - K - 1 : a.n1 - 1;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int64_t rl = rb + r;
+      rofff[r] = (rl < o1 ? o1 : (rl > r_hi ? r_hi : rl)) * a.n2;
+    }
 #pragma unroll
     for (int l = 0; l < K; ++l) {
       const TF* fb = (const TF*)adj.fs[l] + pw * plane;
 #pragma unroll
-      for (int r = 0; r < R; ++r) fv[l][r] = (M)St<TF>::ld((fb + roff[r])[col]);
+      for (int r = 0; r < R; ++r) fv[l][r] = (M)St<TF>::ld((fb + rofff[r])[colf]);
     }
   }
   // One level, in place: row r from level l to l+1 with axis-1 neighbour n1.
@@ -928,7 +941,7 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
         }
         const TF* fb = (const TF*)adj.fs[l] + pn * plane;
 #pragma unroll
-        for (int r = 0; r < R; ++r) fv[l][r] = (M)St<TF>::ld((fb + roff[r])[col]);
+        for (int r = 0; r < R; ++r) fv[l][r] = (M)St<TF>::ld((fb + rofff[r])[colf]);
       }
 #pragma unroll
       for (int r = 0; r < R; ++r) {
