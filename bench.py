@@ -68,13 +68,12 @@ def sync():
 
 # ---------------------------------------------------------------- CPU leg
 
-def cpu_baseline(w, budget_s=12.0):
-    """Time the reference's advect (oracle/_ref) -- or the oracle port when
-    the reference harness was not built -- on this host, in a child process
-    with all its cores, on a sample of the same workload: the same field,
-    velocity and dt, for as many steps as fit in about budget_s."""
-    ncores = len(os.sched_getaffinity(0))
-    env = dict(os.environ, OMP_NUM_THREADS=str(ncores), OMP_PROC_BIND="close",
+def _cpu_advect_run(w, threads, budget_s):
+    """One timed advect of the reference (oracle/_ref) -- or the oracle port
+    when the reference harness was not built -- in a child process with
+    `threads` OpenMP threads, on a sample of the workload (same field,
+    velocity and dt) of as many steps as fit in about budget_s."""
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
                OMP_PLACES="cores")
     code = r"""
 import json, sys, time
@@ -94,10 +93,17 @@ else:
 print(json.dumps(dict(kind=kind, steps=steps, seconds=t, threads=O.lib().orc_num_threads())))
 """ % dict(root=ROOT, args=(list(w.num_cells), list(w.velocity), w.dt, list(w.init_params),
                             w.init_seed, budget_s))
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True,
+                         text=True, timeout=600, check=True).stdout.strip().splitlines()[-1]
+    return json.loads(out)
+
+
+def cpu_baseline(w, budget_s=12.0, single_budget_s=6.0):
+    """The CPU figure on this host: the reference's advect on all its cores
+    (about budget_s of work), plus the same on one core (single_core)."""
+    ncores = len(os.sched_getaffinity(0))
     try:
-        out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True,
-                             text=True, timeout=600, check=True).stdout.strip().splitlines()[-1]
-        r = json.loads(out)
+        r = _cpu_advect_run(w, ncores, budget_s)
     except Exception as e:  # the baseline is reported, never required
         return {"value": None, "unit": "cell-updates/s", "cores": ncores, "kind": None,
                 "sample": f"failed: {e}"}
@@ -107,11 +113,19 @@ print(json.dumps(dict(kind=kind, steps=steps, seconds=t, threads=O.lib().orc_num
             model = next(l.split(":", 1)[1].strip() for l in fh if l.startswith("model name"))
     except Exception:
         pass
-    return {"value": w.ntot * r["steps"] / r["seconds"], "unit": "cell-updates/s",
-            "cores": r["threads"], "kind": r["kind"],
-            "sample": (f"{w.num_cells[0]}^3 x {r['steps']} steps of the same workload "
-                       f"({r['seconds']:.2f} s, advect only, OMP_NUM_THREADS={r['threads']}, "
-                       f"{model})")}
+    out = {"value": w.ntot * r["steps"] / r["seconds"], "unit": "cell-updates/s",
+           "cores": r["threads"], "kind": r["kind"],
+           "sample": (f"{w.num_cells[0]}^3 x {r['steps']} steps of the same workload "
+                      f"({r['seconds']:.2f} s, advect only, OMP_NUM_THREADS={r['threads']}, "
+                      f"{model})")}
+    try:
+        r1 = _cpu_advect_run(w, 1, single_budget_s)
+        out["single_core"] = {"value": w.ntot * r1["steps"] / r1["seconds"],
+                              "sample": (f"{w.num_cells[0]}^3 x {r1['steps']} steps "
+                                         f"({r1['seconds']:.2f} s, OMP_NUM_THREADS=1)")}
+    except Exception as e:
+        out["single_core"] = {"value": None, "sample": f"failed: {e}"}
+    return out
 
 
 # ---------------------------------------------------------------- GPU legs
