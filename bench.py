@@ -441,8 +441,6 @@ def main():
         if dt_name not in only:
             continue
         for k in ks:
-            if dt_name == "bf16" and k > 1:
-                continue  # 2-byte storage always runs one step per launch
             ms, launches, up = run_forward(w, dt_name, a.steps, a.warmup, k)
             cu = w.ntot * a.steps / (ms * 1e-3)
             entry = {"cell_updates_per_s": cu, "ms": ms, "launches": launches,

@@ -1248,9 +1248,6 @@ template <typename TS>
 hipError_t tb_dispatch(const Grid3& g, const TS* in, TS* out, int K, hipStream_t s,
                        void* const* levels = nullptr) {
   using M = typename St<TS>::M;
-  // 2-byte storage: the one-step kernel moves 16 bytes per lane and is
-  // faster than any fused pass with one cell per lane (measured).
-  if (sizeof(TS) < 4) return hipErrorNotSupported;
   const TbCfg cfg = tb_cfg((int)sizeof(TS), K);
   const int R = cfg.R, WY = cfg.WY;
   if (!(g.act[0] && g.act[1] && g.act[2]) || WY * R <= K) return hipErrorNotSupported;
@@ -1439,6 +1436,8 @@ hipError_t launch_fused_pass(int dtype, const Grid3& g, const void* src, void* d
   switch (dtype) {
     case FDB_F64: return tb_dispatch<double>(g, (const double*)src, (double*)dst, k, s, levels);
     case FDB_F32: return tb_dispatch<float>(g, (const float*)src, (float*)dst, k, s, levels);
+    case FDB_BF16:
+      return tb_dispatch<uint16_t>(g, (const uint16_t*)src, (uint16_t*)dst, k, s, levels);
     default: return hipErrorNotSupported;
   }
 }
@@ -1465,7 +1464,10 @@ hipError_t launch_advect_pass(int dtype, const Grid3& g, void* const bufs[2], in
         case FDB_F64: err = tb_dispatch<double>(g, (const double*)src, (double*)dst, K, s); break;
         case FDB_F32: err = tb_dispatch<float>(g, (const float*)src, (float*)dst, K, s); break;
         case FDB_BF16:
-        case FDB_BF16_SR: err = hipErrorNotSupported; break;  // one-step kernel is faster
+          err = tb_dispatch<uint16_t>(g, (const uint16_t*)src, (uint16_t*)dst, K, s);
+          break;
+        // Stochastic rounding draws its bits per (step, cell): one step per launch.
+        case FDB_BF16_SR: err = hipErrorNotSupported; break;
         default: return hipErrorInvalidValue;
       }
       if (err == hipSuccess) {

@@ -38,6 +38,27 @@ def test_time_blocked_fp64_bit_exact(oracle, shape, signs):
         up.close()
 
 
+@pytest.mark.parametrize("shape", SHAPES[:2])
+@pytest.mark.parametrize("signs", list(itertools.product((1, -1), repeat=3)))
+def test_time_blocked_bf16_bit_exact(oracle, shape, signs):
+    """bf16 storage, FP32 arithmetic, every level rounded to nearest even
+    through bf16: the same bits as the oracle's one-step restatement."""
+    from fidibench_amd import Upwind, configs as K
+    v = tuple(s * m for s, m in zip(signs, (0.3, 0.45, 0.2)))
+    dt = K.cfl_dt(shape, v)
+    f0 = oracle.uniform(shape, 0.0, 1.0, seed=sum(shape) + 5)
+    for k in (2, 3, 4, 6, 7, 8):
+        up = Upwind(v, (1, 1, 1), shape, dtype="bf16")
+        up.set_field(f0)
+        start = up.get_field()
+        ref = oracle.advect(start, v, (1, 1, 1), dt, 11)  # uint16 bits: bf16
+        up.set_time_block(k)
+        ms, launches = up.advect_timed(11, dt)
+        assert np.array_equal(bits(up.get_field()), bits(ref)), (shape, signs, k)
+        assert launches < 11  # fused passes actually ran
+        up.close()
+
+
 @pytest.mark.parametrize("dtype", ["f32", "bf16"])
 def test_time_blocked_reduced_precision(oracle, dtype):
     from fidibench_amd import Upwind, configs as K
