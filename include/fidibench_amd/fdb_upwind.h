@@ -112,7 +112,8 @@ int fdb_upwind_init_field(fdb_upwind_t* h, int kind, const double* params, int n
 /* Sum of the field by an FP64 compensated (Neumaier + TwoSum) reduction
  * on the device: the mass used for drift reports. */
 int fdb_upwind_mass(fdb_upwind_t* h, double* out);
-/* Steps fused per HBM pass in advect (temporal blocking); 1 = one kernel
+/* Steps fused per HBM pass in advect (temporal blocking, 1..8; FP64, FP32
+ * and FDB_BF16 -- FDB_BF16_SR always steps one by one); 1 = one kernel
  * launch per step.  Results are identical for every k. */
 int fdb_upwind_set_time_block(fdb_upwind_t* h, int k);
 /* Seed of the stochastic rounding of an FDB_BF16_SR handle (default 0);
