@@ -257,8 +257,8 @@ int run_plan(fdb_tape* t, size_t i, bool forward_only, hipStream_t s, const void
             ++run;
           const int k = (run >= 8 && t->time_block >= 8) ? 8
                         : (run >= 4 && t->time_block >= 4) ? 4 : 0;
-          if (k > 0) {
-            const size_t n = t->stack_steps.size();
+          const size_t n = t->stack_steps.size();
+          if (k > 0 && n - 1 + (size_t)k <= t->slots.size()) {
             void* lv[fdb::kMaxTimeBlock];
             for (int q = 0; q < k; ++q) lv[q] = t->slots[n - 1 + q];
             const hipError_t e =
