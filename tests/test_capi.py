@@ -181,6 +181,20 @@ def test_invalid_arguments_are_reported():
     assert rc == L.ERR_INVALID
 
 
+def test_missing_library_raises(monkeypatch, tmp_path):
+    """No fallback: without the built library every entry point raises."""
+    from fidibench_amd import Filter, Upwind
+    L = _lib()
+    monkeypatch.setattr(L, "LIB_PATH", str(tmp_path / "libfdb_upwind.so"))
+    monkeypatch.setattr(L, "_lib", None)
+    with pytest.raises(FileNotFoundError, match="is missing"):
+        L.lib()
+    with pytest.raises(FileNotFoundError):
+        Upwind((1.0, 1.0, 1.0), (1.0, 1.0, 1.0), (8, 8, 8))
+    with pytest.raises(FileNotFoundError):
+        Filter((8, 8), (0.0, 0.0), (1.0, 1.0), {(0, 0): 1.0})
+
+
 def test_create_without_gpu_is_nodev():
     torch = pytest.importorskip("torch")
     if torch.cuda.device_count() > 0:
