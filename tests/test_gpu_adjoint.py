@@ -194,10 +194,11 @@ def test_checkpointing_is_exact(snaps):
     assert np.allclose(full[2], part[2], rtol=1e-13, atol=0)
 
 
-def _fused_vs_single(shape, v, dt, T, fd, tdt, seed, ld=None, ltdt=None):
+def _fused_vs_single(shape, v, dt, T, fd, tdt, seed, ld=None, ltdt=None, snaps=None):
     """The tape at 8 steps per launch (fused 4-step adjoint passes over the
     stored states) against 1 step per launch; f in fd/tdt, lam in ld/ltdt
-    (default the same)."""
+    (default the same); snaps = None stores every state, else the memory
+    budget leaves that many snapshots (recomputation)."""
     from fidibench_amd.autograd import Tape
     ld = fd if ld is None else ld
     ltdt = tdt if ltdt is None else ltdt
@@ -205,8 +206,11 @@ def _fused_vs_single(shape, v, dt, T, fd, tdt, seed, ld=None, ltdt=None):
     f0 = torch.rand(shape, generator=g, dtype=torch.float64).to(tdt).cuda()
     lamT = torch.randn(shape, generator=g, dtype=torch.float64).to(ltdt).cuda()
     out = []
+    fbytes = f0.numel() * f0.element_size()
+    budget = 1 << 31 if snaps is None else 3 * fbytes + lamT.numel() * lamT.element_size() + \
+        snaps * fbytes
     for k in (8, 1):
-        tape = Tape(shape, v, (1, 1, 1), dt, T, fd, ld, 1 << 31)
+        tape = Tape(shape, v, (1, 1, 1), dt, T, fd, ld, budget)
         tape.set_time_block(k)
         fT = torch.empty_like(f0)
         tape.forward(f0, fT)
@@ -214,7 +218,8 @@ def _fused_vs_single(shape, v, dt, T, fd, tdt, seed, ld=None, ltdt=None):
         gv = tape.backward(lamT, lam0)
         st = tape.stats()
         tape.close()
-        assert st["adjoint_steps"] == T and st["snapshots"] == T - 1
+        assert st["adjoint_steps"] == T
+        assert st["snapshots"] == (T - 1 if snaps is None else snaps)
         out.append((fT.cpu(), lam0.cpu(), np.array(gv)))
     (f8, l8, g8), (f1, l1, g1) = out
     assert torch.equal(f8.view(torch.uint8), f1.view(torch.uint8))
@@ -242,6 +247,20 @@ def test_fused_adjoint_fp32_over_bf16_states(fkind, v):
     fd = L.BF16 if fkind == "bf16" else L.BF16_SR
     _fused_vs_single((20, 70, 130), v, 0.003, 13, fd, torch.bfloat16, seed=3, ld=L.F32,
                      ltdt=torch.float32)
+
+
+def test_fused_adjoint_grid_smaller_than_a_tile():
+    from fidibench_amd import _lib as L
+    _fused_vs_single((5, 9, 17), (-0.45, 0.3, -0.2), 0.004, 9, L.F64, torch.float64, seed=11)
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_fused_passes_within_checkpointing(dtype):
+    """Few snapshots: store-all and fused adjoint runs interleave with
+    recomputation; same bits as one step per launch."""
+    from fidibench_amd import _lib as L
+    fd, tdt = (L.F64, torch.float64) if dtype == "f64" else (L.F32, torch.float32)
+    _fused_vs_single((20, 36, 70), (0.5, -0.3, 0.35), 0.003, 30, fd, tdt, seed=12, snaps=6)
 
 
 def test_fused_adjoint_long_run():
