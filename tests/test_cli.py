@@ -3,7 +3,7 @@
 
 The reference prints two OpenMP banner lines that a GPU run has no use for;
 they are removed from the golden text before comparing.  Help output is
-compared option by option (this build adds -precision and -timeBlock and
+compared option by option (upwindCxx here adds -precision and -timeBlock and
 its own footer).  The reference's CTest pass rule (upwind/cxx/
 CMakeLists.txt:54-91) is applied too.
 """
