@@ -169,12 +169,19 @@ def _tape_grad(f0, target, v, dt, T, max_bytes=None, f_dtype=torch.float64):
     return float(loss.detach()), f0t.grad.double().cpu().numpy(), vt.grad.numpy().copy(), fT
 
 
-def test_gradient_matches_oracle(oracle):
-    shape, v, dt, T, f0, target = _setup()
+# 3-D cases run the fused store-all forward and fused adjoint passes (T = 11
+# leaves 10 stored states); 2-D and 1-D grids take the one-step kernels.
+@pytest.mark.parametrize("shape,v,T", [((10, 12, 70), (0.45, -0.3, 0.2), 6),
+                                       ((12, 40, 100), (-0.4, 0.3, -0.25), 11),
+                                       ((24, 70), (0.5, -0.35), 9),
+                                       ((130,), (-0.6,), 9)])
+def test_gradient_matches_oracle(oracle, shape, v, T):
+    shape, v, dt, T, f0, target = _setup(shape=shape, v=v, T=T)
+    lengths = (1.0,) * len(shape)
     loss, gf, gv, fT = _tape_grad(f0, target, v, dt, T)
-    rl, rgf, rgv = oracle.misfit_grad(f0, target, v, (1, 1, 1), dt, T)
+    rl, rgf, rgv = oracle.misfit_grad(f0, target, v, lengths, dt, T)
     assert np.array_equal(bits(fT.detach().cpu().numpy()),
-                          bits(oracle.advect(f0, v, (1, 1, 1), dt, T)))
+                          bits(oracle.advect(f0, v, lengths, dt, T)))
     assert abs(loss - rl) <= 1e-14 * rl
     assert np.array_equal(bits(gf), bits(rgf))  # same adjoint op order: bit-exact
     assert np.allclose(gv, rgv, rtol=1e-12, atol=0)
