@@ -38,6 +38,8 @@ SIGNATURES = {
     "fdb_upwind_std": (_i, [_vp, _dp]),
     "fdb_upwind_print": (_i, [_vp]),
     "fdb_upwind_save_vtk": (_i, [_vp, C.c_char_p]),
+    "fdb_upwind_save_raw": (_i, [_vp, C.c_char_p]),
+    "fdb_upwind_load_raw": (_i, [_vp, C.c_char_p]),
     "fdb_upwind_ntot": (_i, [_vp, _u64p]),
     "fdb_upwind_dtype": (_i, [_vp, C.POINTER(_i)]),
     "fdb_upwind_set_field": (_i, [_vp, _vp, _i]),

@@ -335,6 +335,81 @@ int fdb_upwind_save_vtk(fdb_upwind_t* h, const char* filename) {
   return fdb_host_save_vtk(h->ndims, h->n, h->len, f.data(), filename);
 }
 
+// Raw field file: this 64-byte header, then the field's ntot values in the
+// handle's storage type (FDB_BF16_SR as bf16 bits), little-endian,
+// row-major with the last axis contiguous -- numpy.fromfile(name, dtype,
+// offset=64) reads it.  steps_done and seed let a stochastic-rounding run
+// resume with the same random bits.
+struct RawHeader {
+  char magic[8];  // "FDBFLD01"
+  uint32_t ndims, dtype;
+  uint64_t n[3];  // axes 0..ndims-1; the rest 1
+  uint64_t steps_done, seed;
+  uint64_t reserved;
+};
+static_assert(sizeof(RawHeader) == 64, "raw header is 64 bytes");
+constexpr char kRawMagic[8] = {'F', 'D', 'B', 'F', 'L', 'D', '0', '1'};
+
+int fdb_upwind_save_raw(fdb_upwind_t* h, const char* filename) {
+  if (!h || !filename) return fail(FDB_ERR_INVALID, "save_raw: null argument");
+  std::unique_ptr<FILE, int (*)(FILE*)> fp(std::fopen(filename, "wb"), &std::fclose);
+  if (!fp) return fail(FDB_ERR_IO, std::string("save_raw: cannot open ") + filename);
+  RawHeader hd{};
+  std::memcpy(hd.magic, kRawMagic, sizeof(kRawMagic));
+  hd.ndims = (uint32_t)h->ndims;
+  hd.dtype = (uint32_t)h->dtype;
+  for (int j = 0; j < 3; ++j) hd.n[j] = j < h->ndims ? h->n[j] : 1;
+  hd.steps_done = h->steps_done;
+  hd.seed = h->seed;
+  if (std::fwrite(&hd, sizeof(hd), 1, fp.get()) != 1)
+    return fail(FDB_ERR_IO, "save_raw: write failed");
+  const uint64_t per = kStageBytes / h->esz;
+  const char* src = (const char*)h->buf[h->cur];
+  for (uint64_t i0 = 0; i0 < h->ntot; i0 += per) {
+    const uint64_t cnt = std::min<uint64_t>(per, h->ntot - i0);
+    FDB_HIP(hipMemcpyAsync(h->stage, src + i0 * h->esz, cnt * h->esz, hipMemcpyDeviceToHost,
+                           h->stream));
+    FDB_HIP(hipStreamSynchronize(h->stream));
+    if (std::fwrite(h->stage, h->esz, cnt, fp.get()) != cnt)
+      return fail(FDB_ERR_IO, "save_raw: write failed");
+  }
+  if (std::fflush(fp.get()) != 0) return fail(FDB_ERR_IO, "save_raw: write failed");
+  return FDB_OK;
+}
+
+int fdb_upwind_load_raw(fdb_upwind_t* h, const char* filename) {
+  if (!h || !filename) return fail(FDB_ERR_INVALID, "load_raw: null argument");
+  std::unique_ptr<FILE, int (*)(FILE*)> fp(std::fopen(filename, "rb"), &std::fclose);
+  if (!fp) return fail(FDB_ERR_IO, std::string("load_raw: cannot open ") + filename);
+  RawHeader hd{};
+  if (std::fread(&hd, sizeof(hd), 1, fp.get()) != 1 ||
+      std::memcmp(hd.magic, kRawMagic, sizeof(kRawMagic)) != 0)
+    return fail(FDB_ERR_IO, "load_raw: not a raw field file");
+  bool same = hd.ndims == (uint32_t)h->ndims && hd.dtype == (uint32_t)h->dtype;
+  for (int j = 0; j < 3 && same; ++j) same = hd.n[j] == (j < h->ndims ? h->n[j] : 1);
+  if (!same)
+    return fail(FDB_ERR_INVALID, "load_raw: the file's shape or storage type differs");
+  // The whole payload must be there (and nothing after it).
+  if (std::fseek(fp.get(), 0, SEEK_END) != 0 ||
+      std::ftell(fp.get()) != (long)(sizeof(hd) + h->ntot * h->esz) ||
+      std::fseek(fp.get(), (long)sizeof(hd), SEEK_SET) != 0)
+    return fail(FDB_ERR_IO, "load_raw: file size does not match its header");
+  const uint64_t per = kStageBytes / h->esz;
+  char* dst = (char*)h->buf[h->cur];
+  for (uint64_t i0 = 0; i0 < h->ntot; i0 += per) {
+    const uint64_t cnt = std::min<uint64_t>(per, h->ntot - i0);
+    FDB_HIP(hipStreamSynchronize(h->stream));  // staging buffer free
+    if (std::fread(h->stage, h->esz, cnt, fp.get()) != cnt)
+      return fail(FDB_ERR_IO, "load_raw: read failed");
+    FDB_HIP(hipMemcpyAsync(dst + i0 * h->esz, h->stage, cnt * h->esz, hipMemcpyHostToDevice,
+                           h->stream));
+  }
+  FDB_HIP(hipStreamSynchronize(h->stream));
+  h->steps_done = hd.steps_done;
+  h->seed = hd.seed;
+  return FDB_OK;
+}
+
 int fdb_upwind_ntot(const fdb_upwind_t* h, uint64_t* out) {
   if (!h || !out) return fail(FDB_ERR_INVALID, "ntot: null argument");
   *out = h->ntot;

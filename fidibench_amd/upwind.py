@@ -78,6 +78,14 @@ class Upwind:
         L.check(L.lib().fdb_upwind_save_vtk(self._h, str(filename).encode()))
 
     # -- extensions
+    def save_raw(self, filename: str) -> None:
+        """Binary dump (64-byte header, then the storage-type values)."""
+        L.check(L.lib().fdb_upwind_save_raw(self._h, str(filename).encode()))
+
+    def load_raw(self, filename: str) -> None:
+        """Resume from save_raw's file: field, step count and seed."""
+        L.check(L.lib().fdb_upwind_load_raw(self._h, str(filename).encode()))
+
     @property
     def ntot(self) -> int:
         n = C.c_uint64()

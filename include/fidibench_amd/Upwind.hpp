@@ -73,6 +73,13 @@ class Upwind {
   void saveVTK(const std::string& filename) {
     detail::check(fdb_upwind_save_vtk(h_, filename.c_str()), "saveVTK");
   }
+  // Extensions: binary dump and resume (fdb_upwind_save_raw / load_raw).
+  void saveRaw(const std::string& filename) {
+    detail::check(fdb_upwind_save_raw(h_, filename.c_str()), "saveRaw");
+  }
+  void loadRaw(const std::string& filename) {
+    detail::check(fdb_upwind_load_raw(h_, filename.c_str()), "loadRaw");
+  }
 
   double checksum() const {
     double r = 0.0;

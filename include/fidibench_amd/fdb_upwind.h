@@ -93,6 +93,17 @@ int fdb_upwind_print(fdb_upwind_t* h);
  * with the same bytes as the reference writer. */
 int fdb_upwind_save_vtk(fdb_upwind_t* h, const char* filename);
 
+/* Fast binary dump and resume (extension; the reference writes VTK only):
+ * a 64-byte header ("FDBFLD01", ndims, dtype, axes, steps done, seed) and
+ * the field in its storage type, little-endian, last axis contiguous
+ * (numpy.fromfile(name, dtype, offset=64)).  load_raw restores the field
+ * and the step count/seed of the stochastic rounding into a handle of the
+ * same shape and storage type, so a run can stop and resume bit-exactly;
+ * FDB_ERR_INVALID for another shape or type, FDB_ERR_IO for a missing,
+ * foreign or truncated file. */
+int fdb_upwind_save_raw(fdb_upwind_t* h, const char* filename);
+int fdb_upwind_load_raw(fdb_upwind_t* h, const char* filename);
+
 /* -- extensions of the handle (no member in the reference) -- */
 
 /* Number of cells (the reference's private ntot, upwind.cxx:120). */
