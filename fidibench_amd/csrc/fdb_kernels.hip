@@ -785,6 +785,34 @@ struct TbAux {
   double* partials;
 };
 
+// Raw buffer access to one row: the row's start (uniform) goes into the
+// resource, the lane's byte offset into a VGPR, so no lane does 64-bit
+// address arithmetic.  Offsets stay below the row length (no access is
+// dropped); stores are nontemporal (aux 2).
+template <typename T>
+__device__ inline __amdgpu_buffer_rsrc_t row_rsrc(const T* row, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(row), 0, bytes, 0x00020000);
+}
+using u32x2_t = uint32_t __attribute__((ext_vector_type(2)));
+__device__ inline double buf_ld(__amdgpu_buffer_rsrc_t rs, uint32_t off, const double*) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0));
+}
+__device__ inline float buf_ld(__amdgpu_buffer_rsrc_t rs, uint32_t off, const float*) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0));
+}
+__device__ inline uint16_t buf_ld(__amdgpu_buffer_rsrc_t rs, uint32_t off, const uint16_t*) {
+  return __builtin_amdgcn_raw_buffer_load_b16(rs, off, 0, 0);
+}
+__device__ inline void buf_st_nt(double v, __amdgpu_buffer_rsrc_t rs, uint32_t off) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2_t, v), rs, off, 0, 2);
+}
+__device__ inline void buf_st_nt(float v, __amdgpu_buffer_rsrc_t rs, uint32_t off) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), rs, off, 0, 2);
+}
+__device__ inline void buf_st_nt(uint16_t v, __amdgpu_buffer_rsrc_t rs, uint32_t off) {
+  __builtin_amdgcn_raw_buffer_store_b16(v, rs, off, 0, 2);
+}
+
 // x mod n in [0, n) for any x (n > 0).
 __device__ inline int64_t pmod(int64_t x, int64_t n) {
   x %= n;
@@ -816,6 +844,8 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
   const int64_t cl = LO2 ? o2 - K + lane : o2 + lane;  // unwrapped column of this lane
   const uint32_t col = (uint32_t)pmod(cl, a.n2);
   const bool col_out = (LO2 ? lane >= K : lane < 64 - K) && cl >= 0 && cl < a.n2;
+  const uint32_t cb = col * (uint32_t)sizeof(TS);         // byte offset in the row
+  const int rbytes = (int)(a.n2 * (int64_t)sizeof(TS));  // row length (< 2^31)
   // Rows of this wave: block rows w*R .. w*R+R-1 of the extended tile,
   // whose output rows are o1 .. o1+WY*R-1-K.
   const int64_t o1 = t1 * (WY * R - K);
@@ -847,7 +877,7 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
   {
     const TS* base = in + pw * plane;
 #pragma unroll
-    for (int r = 0; r < R; ++r) x[r] = St<TS>::ld((base + roff[r])[col]);
+    for (int r = 0; r < R; ++r) x[r] = St<TS>::ld(buf_ld(row_rsrc(base + roff[r], rbytes), cb, in));
 #pragma unroll
     for (int r = 0; r < R; ++r) settle(x[r]);
   }
@@ -877,15 +907,18 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
       for (int r = 0; r < R; ++r) fv[l][r] = (M)St<TF>::ld((fb + rofff[r])[colf]);
     }
   }
-  // One level, in place: row r from level l to l+1 with axis-1 neighbour n1.
+  // One plane, all K levels; each level in place: row r from level l to
+  // l+1 with axis-1 neighbour n1.  For K <= 6 (forward modes) the plane
+  // loop runs two planes per trip so that the registers of prev[l] and of
+  // level l alternate between planes instead of being copied.
   int lvl = 0;
-  for (int64_t m = 0; m < cnt; ++m) {
+  auto plane_step = [&](const int64_t m) __attribute__((always_inline)) {
     int64_t pn = pw + step;
     pn = pn < 0 ? a.n0 - 1 : (pn == a.n0 ? 0 : pn);
     if (m + 1 < cnt) {
       const TS* base = in + pn * plane;
 #pragma unroll
-      for (int r = 0; r < R; ++r) nx[r] = St<TS>::ld((base + roff[r])[col]);
+      for (int r = 0; r < R; ++r) nx[r] = St<TS>::ld(buf_ld(row_rsrc(base + roff[r], rbytes), cb, in));
     }
     double ps[3] = {0.0, 0.0, 0.0};  // PROD: this plane's products, all levels
 #pragma unroll
@@ -963,7 +996,7 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
           TS* ob = (TS*)adj.lv[l] + pw * plane;
 #pragma unroll
           for (int r = 0; r < R; ++r)
-            if (row_out[r]) __builtin_nontemporal_store(St<TS>::st(x[r]), (ob + roff[r]) + col);
+            if (row_out[r]) buf_st_nt(St<TS>::st(x[r]), row_rsrc(ob + roff[r], rbytes), cb);
         }
       }
     }
@@ -975,13 +1008,24 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
       TS* ob = out + pw * plane;
 #pragma unroll
       for (int r = 0; r < R; ++r)
-        if (row_out[r]) __builtin_nontemporal_store(St<TS>::st(x[r]), (ob + roff[r]) + col);
+        if (row_out[r]) buf_st_nt(St<TS>::st(x[r]), row_rsrc(ob + roff[r], rbytes), cb);
     }
     if (m + 1 < cnt) {
 #pragma unroll
       for (int r = 0; r < R; ++r) x[r] = nx[r];
     }
     pw = pn;
+  };
+  int64_t m = 0;
+  if constexpr (K <= 6 && !PROD) {
+    for (; m + 1 < cnt; m += 2) {
+      plane_step(m);
+      plane_step(m + 1);
+    }
+    if (m < cnt) plane_step(m);
+  } else {  // K = 7, 8 and the adjoint: two planes per trip would spill
+            // (FP64, adjoint) or lose a wave per SIMD (FP32, BF16)
+    for (; m < cnt; ++m) plane_step(m);
   }
   if constexpr (PROD) block_reduce_vacc<64 * WY>(acc, adj.partials);
 }
@@ -1256,6 +1300,9 @@ hipError_t tb_dispatch(const Grid3& g, const TS* in, TS* out, int K, hipStream_t
   a.n1 = g.n[1];
   a.n2 = g.n[2];
   a.u0 = g.up[0];
+  // Rows are addressed by 32-bit byte offsets; longer rows take one-step
+  // launches.
+  if (a.n2 * (int64_t)sizeof(TS) > INT32_MAX) return hipErrorNotSupported;
   a.tiles2 = (a.n2 + (64 - K) - 1) / (64 - K);
   a.tiles1 = (a.n1 + (WY * R - K) - 1) / (WY * R - K);
   const int64_t tiles = a.tiles2 * a.tiles1;
@@ -1324,6 +1371,7 @@ hipError_t tb_adj_dispatch(const Grid3& g, const TS* in, TS* out, const void* co
   a.n1 = g.n[1];
   a.n2 = g.n[2];
   a.u0 = -g.up[0];  // the transpose gathers from the downwind side
+  if (a.n2 * (int64_t)sizeof(TS) > INT32_MAX) return hipErrorNotSupported;  // row offsets
   a.tiles2 = (a.n2 + (64 - K) - 1) / (64 - K);
   a.tiles1 = (a.n1 + (WY * R - K) - 1) / (WY * R - K);
   const int64_t tiles = a.tiles2 * a.tiles1;
