@@ -29,23 +29,34 @@ __host__ __device__ inline float bf16_to_f32(uint16_t h) {
 
 // Storage type TS and the type arithmetic is done in (M).
 template <typename TS> struct St;
+// rnd(x) == ld(st(x)): x rounded to the storage type, kept in M.
 template <> struct St<double> {
   using M = double;
   __device__ static M ld(double x) { return x; }
   __device__ static double st(M x) { return x; }
+  __device__ static M rnd(M x) { return x; }
 };
 template <> struct St<float> {
   using M = float;
   __device__ static M ld(float x) { return x; }
   __device__ static float st(M x) { return x; }
+  __device__ static M rnd(M x) { return x; }
 };
 // bf16 rounding on the device is gfx950's v_cvt_pk_bf16_f32: round to
 // nearest, ties to even, the rule the host side (fdb_capi.cpp) and the
-// oracle implement in software.
+// oracle implement in software.  rnd converts the pair (+0, x): the bf16
+// of x lands in the high half over sixteen zero bits, which is the binary32
+// value of that bf16 -- one instruction instead of a convert and a shift.
 template <> struct St<uint16_t> {
   using M = float;
+  typedef float f2 __attribute__((ext_vector_type(2)));
+  typedef __bf16 b2 __attribute__((ext_vector_type(2)));
   __device__ static M ld(uint16_t x) { return bf16_to_f32(x); }
   __device__ static uint16_t st(M x) { return __builtin_bit_cast(uint16_t, (__bf16)x); }
+  __device__ static M rnd(M x) {
+    const f2 v = {0.0f, x};
+    return __builtin_bit_cast(float, __builtin_convertvector(v, b2));
+  }
 };
 
 // Stochastic rounding of step results for bf16 storage (FDB_BF16_SR): 16
@@ -989,7 +1000,7 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
         g = g - d1[r];
         g = g - d2[r];
         prev[l][r] = x[r];
-        x[r] = St<TS>::ld(St<TS>::st(g));  // each level as stored
+        x[r] = St<TS>::rnd(g);  // each level as stored
       }
       if constexpr (ALL) {
         if (l + 1 < K && m >= K && col_out) {  // level l+1 (level K below)
