@@ -85,3 +85,29 @@ def test_time_blocked_large_checksum(manifest):
     ms, launches = up.advect_timed(w.steps, w.dt)
     assert launches == 25
     assert repr(up.checksum()) == m["checksum"]
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32", "bf16"])
+@pytest.mark.parametrize("k", [4, 6])
+@pytest.mark.parametrize("n0", [7, 8])
+@pytest.mark.parametrize("blocks", ["1", "100000"])
+def test_two_plane_march_both_parities(oracle, monkeypatch, dtype, k, n0, blocks):
+    """K <= 6 marches two planes per loop trip with a one-plane tail.  A
+    pass marches chunk + K planes; one chunk (chunk = n0) or one plane per
+    chunk (chunk = 1) gives both an even and an odd count across these
+    cases, and every one must equal the oracle bit for bit."""
+    from fidibench_amd import Upwind, configs as K
+    size = {"f64": 8, "f32": 4, "bf16": 2}[dtype]
+    monkeypatch.setenv(f"FDB_TB_CFG_{size}_{k}", f"9,{8 if k >= 6 else 4},{blocks}")
+    shape = (n0, 30, 70)
+    v = (-0.35, 0.4, -0.25)
+    dt = K.cfl_dt(shape, v)
+    up = Upwind(v, (1, 1, 1), shape, dtype=dtype)
+    up.set_field(oracle.uniform(shape, 0.0, 1.0, seed=n0 + k))
+    start = up.get_field()
+    ref = oracle.advect(start, v, (1, 1, 1), dt, 2 * k + 1)
+    up.set_time_block(k)
+    ms, launches = up.advect_timed(2 * k + 1, dt)
+    assert launches < 2 * k + 1
+    assert np.array_equal(bits(up.get_field()), bits(ref)), (dtype, k, n0, blocks)
+    up.close()
