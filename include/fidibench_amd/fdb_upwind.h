@@ -100,7 +100,9 @@ int fdb_upwind_save_vtk(fdb_upwind_t* h, const char* filename);
  * and the step count/seed of the stochastic rounding into a handle of the
  * same shape and storage type, so a run can stop and resume bit-exactly;
  * FDB_ERR_INVALID for another shape or type, FDB_ERR_IO for a missing,
- * foreign or truncated file. */
+ * foreign or truncated file.  These checks come before any data is read
+ * and leave the handle unchanged; a read error after them (a file that
+ * shrinks while it is read) leaves the field partly replaced. */
 int fdb_upwind_save_raw(fdb_upwind_t* h, const char* filename);
 int fdb_upwind_load_raw(fdb_upwind_t* h, const char* filename);
 
