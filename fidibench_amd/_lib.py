@@ -18,6 +18,8 @@ ITEMSIZE = {F64: 8, F32: 4, BF16: 2, BF16_SR: 2}
 
 INIT_DELTA, INIT_SPHERE, INIT_GAUSS, INIT_GAUSS_SUM, INIT_UNIFORM = range(5)
 CK_ADVANCE, CK_STORE, CK_RESTORE, CK_POP, CK_ADJOINT = 1, 2, 3, 4, 5
+GRAD_CHECKPOINT, GRAD_SHIFT = 0, 1
+GRAD_METHODS = {"checkpoint": GRAD_CHECKPOINT, "shift": GRAD_SHIFT}
 
 OK = 0
 ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_IO, ERR_NODEV = -1, -2, -3, -4, -5
@@ -52,6 +54,8 @@ SIGNATURES = {
     "fdb_upwind_advect_timed": (_i, [_vp, _i, C.c_double, C.POINTER(C.c_float), C.POINTER(_i)]),
     "fdb_advect_dev": (_i, [_i, _u64p, _dp, _dp, C.c_double, _i, _vp, _vp, _i, _i, _vp,
                             C.POINTER(_i)]),
+    "fdb_advect_dev_seeded": (_i, [_i, _u64p, _dp, _dp, C.c_double, _i, _vp, _vp, _i, _i,
+                                   C.c_uint64, C.c_uint64, _vp, C.POINTER(_i)]),
     "fdb_adjoint_step_dev": (_i, [_i, _u64p, _dp, _dp, C.c_double, _i, _vp, _vp, _i, _vp, _vp,
                                   _vp]),
     "fdb_sum_dev": (_i, [_i, _vp, C.c_uint64, _dp, _vp]),
@@ -62,6 +66,7 @@ SIGNATURES = {
     "fdb_tape_backward": (_i, [_vp, _vp, _vp, _dp, _vp]),
     "fdb_tape_stats": (_i, [_vp, C.POINTER(_i), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "fdb_tape_set_time_block": (_i, [_vp, _i]),
+    "fdb_tape_set_method": (_i, [_vp, _i]),
     "fdb_tape_destroy": (_i, [_vp]),
     "fdb_init_field_dev": (_i, [_i, _u64p, _dp, _i, _i, _dp, _i, C.c_uint64, _vp, _vp]),
     "fdb_compare_dev": (_i, [_i, _vp, _i, _vp, C.c_uint64, _dp, _vp]),

@@ -5,11 +5,17 @@
     loss.backward()                      # d loss / d f0 and d loss / d velocity
 
 The forward runs nsteps of the upwind step (libfdb_upwind.so, the same
-kernels as Upwind.advect) with dt held fixed, and keeps checkpoints on a
-tape (fdb_tape_*).  backward() runs the adjoint kernel -- the exact
-transpose of the step -- over the recomputed states (binomial checkpointing
-within `max_bytes` of device memory) and the block-reduced velocity
-gradient dL/dv_j = -(dt*u_j/dx_j) * sum_t <lam_{t+1}, S_j f_t - f_t>.
+kernels as Upwind.advect) with dt held fixed, on a tape (fdb_tape_*).
+backward() runs the adjoint kernel -- the exact transpose of the step --
+and the block-reduced velocity gradient
+dL/dv_j = -(dt*u_j/dx_j) * sum_t <lam_{t+1}, S_j f_t - f_t>, by one of two
+methods (fdb_tape_set_method):
+  "shift" (default)  the step and the axis shifts are circulant and
+      commute, so every term of the sum equals the t = 0 one: T fused
+      transpose steps and one product step against f0; no stored states;
+  "checkpoint"  every step's products against its own forward state, the
+      states recomputed from binomial checkpoints within `max_bytes`.
+Both give the same dL/df0 bit for bit and dL/dv equal up to rounding.
 f0 may be float64 (the reference's type), float32 or bfloat16; the adjoint
 runs in float64 for float64 fields and in float32 otherwise.
 """
@@ -35,7 +41,7 @@ class Tape:
 
     def __init__(self, shape: Sequence[int], velocity: Sequence[float],
                  lengths: Sequence[float], dt: float, nsteps: int, f_dtype: int,
-                 lam_dtype: int, max_bytes: int):
+                 lam_dtype: int, max_bytes: int, method: str = "checkpoint"):
         self.shape = tuple(int(x) for x in shape)
         self.ndims = len(self.shape)
         self.f_dtype, self.lam_dtype = f_dtype, lam_dtype
@@ -44,9 +50,14 @@ class Tape:
                                         L.dbls(lengths), float(dt), int(nsteps), f_dtype,
                                         lam_dtype, int(max_bytes), C.byref(h)))
         self._h = h
+        self.method = method
+        L.check(L.lib().fdb_tape_set_method(h, L.GRAD_METHODS[method]))
+        self._f0 = None
 
     def forward(self, f0: torch.Tensor, fT: torch.Tensor) -> None:
         L.check(L.lib().fdb_tape_forward(self._h, f0.data_ptr(), fT.data_ptr(), _stream()))
+        # "shift" reads f0 again in backward: keep the tensor alive until then.
+        self._f0 = f0 if self.method == "shift" else None
 
     def backward(self, lamT: torch.Tensor, lam0: torch.Tensor) -> list:
         gv = (C.c_double * 3)()
@@ -68,6 +79,7 @@ class Tape:
         if getattr(self, "_h", None) and self._h.value:
             L.lib().fdb_tape_destroy(self._h)
             self._h = C.c_void_p()
+        self._f0 = None
 
     def __del__(self):
         try:
@@ -83,7 +95,7 @@ def default_budget(fraction: float = 0.85) -> int:
 
 class UpwindAdvect(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, f0, velocity, dt, nsteps, lengths, max_bytes, lam_dtype):
+    def forward(ctx, f0, velocity, dt, nsteps, lengths, max_bytes, lam_dtype, method):
         if not f0.is_cuda:
             raise ValueError("f0 must be a GPU tensor")
         f_dtype = _TORCH_TO_FDB[f0.dtype]
@@ -100,8 +112,8 @@ class UpwindAdvect(torch.autograd.Function):
                                            int(nsteps), 1, _stream(), C.byref(res)))
             ctx.tape = None
             return b if res.value else a
-        tape = Tape(f0c.shape, v, lengths, dt, nsteps, f_dtype, lam,
-                    max_bytes if max_bytes is not None else default_budget())
+        budget = max_bytes if max_bytes is not None else default_budget()
+        tape = Tape(f0c.shape, v, lengths, dt, nsteps, f_dtype, lam, budget, method=method)
         fT = torch.empty_like(f0c)
         tape.forward(f0c, fT)
         ctx.tape = tape
@@ -122,16 +134,19 @@ class UpwindAdvect(torch.autograd.Function):
         g_f0 = lam0.to(ctx.f_dtype) if ctx.needs_input_grad[0] else None
         g_v = (torch.tensor(gv, dtype=torch.float64).to(vdtype).to(vdev)
                if ctx.needs_input_grad[1] else None)
-        return g_f0, g_v, None, None, None, None, None
+        return g_f0, g_v, None, None, None, None, None, None
 
 
 def advect(f0: torch.Tensor, velocity: torch.Tensor, dt: float, nsteps: int,
            lengths: Sequence[float] = (1.0, 1.0, 1.0), max_bytes: int | None = None,
-           lam_dtype: int | None = None) -> torch.Tensor:
-    """f_T after nsteps upwind steps from f0 (differentiable in f0 and velocity)."""
+           lam_dtype: int | None = None, method: str = "shift") -> torch.Tensor:
+    """f_T after nsteps upwind steps from f0 (differentiable in f0 and
+    velocity); method "shift" or "checkpoint" (module docstring)."""
     nd = f0.dim()
+    if method not in L.GRAD_METHODS:
+        raise ValueError(f"method must be one of {sorted(L.GRAD_METHODS)}")
     return UpwindAdvect.apply(f0, velocity, float(dt), int(nsteps), tuple(lengths)[:nd],
-                              max_bytes, lam_dtype)
+                              max_bytes, lam_dtype, method)
 
 
 class _Misfit(torch.autograd.Function):

@@ -130,6 +130,13 @@ int dev_scratch(size_t bytes, double** out) {
   return FDB_OK;
 }
 
+// A malformed tuning variable is an argument error of every launching
+// entry point (reported before any device work).
+int check_tuning() {
+  if (const char* m = fdb::tuning_error()) return fail(FDB_ERR_INVALID, m);
+  return FDB_OK;
+}
+
 int check_device() {
   int n = 0;
   hipError_t e = hipGetDeviceCount(&n);
@@ -235,6 +242,9 @@ int fdb_upwind_create(int ndims, const double* velocity, const double* lengths,
       return fail(FDB_ERR_INVALID, "create: too many cells");
     ntot *= num_cells[j];
   }
+  if (!fdb::fits_bytes(ntot, fdb::dtype_size(dtype)))
+    return fail(FDB_ERR_INVALID, "create: field size in bytes overflows");
+  if (int rc = check_tuning()) return rc;
   if (int rc = check_device()) return rc;
   std::unique_ptr<fdb_upwind, int (*)(fdb_upwind*)> h(new fdb_upwind, release);
   h->ndims = ndims;
@@ -503,9 +513,10 @@ int fdb_upwind_set_time_block(fdb_upwind_t* h, int k) {
 
 // ------------------------------------------------------- device pointers
 
-int fdb_advect_dev(int ndims, const uint64_t* num_cells, const double* velocity,
-                   const double* lengths, double delta_time, int dtype, void* a, void* b,
-                   int nsteps, int time_block, void* stream, int* result_in_b) {
+int fdb_advect_dev_seeded(int ndims, const uint64_t* num_cells, const double* velocity,
+                          const double* lengths, double delta_time, int dtype, void* a, void* b,
+                          int nsteps, int time_block, uint64_t seed, uint64_t step0,
+                          void* stream, int* result_in_b) {
   fdb::Grid3 g;
   if (!fdb::make_grid(ndims, num_cells, velocity, lengths, delta_time, &g))
     return fail(FDB_ERR_INVALID, "advect_dev: invalid geometry");
@@ -513,16 +524,27 @@ int fdb_advect_dev(int ndims, const uint64_t* num_cells, const double* velocity,
   if (!a || !b || a == b) return fail(FDB_ERR_INVALID, "advect_dev: need two distinct buffers");
   if (time_block < 1 || time_block > fdb::kMaxTimeBlock)
     return fail(FDB_ERR_INVALID, "advect_dev: time_block out of range");
+  if (int rc = check_tuning()) return rc;
   void* const bufs[2] = {a, b};
   int cur = 0;
   int left = nsteps;
+  uint64_t stepno = step0;  // keys the stochastic rounding of each step
   while (left > 0) {
     const int k = std::min(left, time_block);
-    FDB_HIP(fdb::launch_advect_pass(dtype, g, bufs, &cur, k, (hipStream_t)stream, nullptr));
+    FDB_HIP(fdb::launch_advect_pass(dtype, g, bufs, &cur, k, (hipStream_t)stream, nullptr,
+                                    stepno, seed));
+    stepno += (uint64_t)k;
     left -= k;
   }
   if (result_in_b) *result_in_b = cur;
   return FDB_OK;
+}
+
+int fdb_advect_dev(int ndims, const uint64_t* num_cells, const double* velocity,
+                   const double* lengths, double delta_time, int dtype, void* a, void* b,
+                   int nsteps, int time_block, void* stream, int* result_in_b) {
+  return fdb_advect_dev_seeded(ndims, num_cells, velocity, lengths, delta_time, dtype, a, b,
+                               nsteps, time_block, 0, 0, stream, result_in_b);
 }
 
 int fdb_adjoint_step_dev(int ndims, const uint64_t* num_cells, const double* velocity,
@@ -538,6 +560,7 @@ int fdb_adjoint_step_dev(int ndims, const uint64_t* num_cells, const double* vel
   if (!lam_next || !lam || lam_next == lam)
     return fail(FDB_ERR_INVALID, "adjoint_step_dev: need two distinct buffers");
   if (f_t && !vgrad_acc) return fail(FDB_ERR_INVALID, "adjoint_step_dev: vgrad_acc is null");
+  if (int rc = check_tuning()) return rc;
   double* part = nullptr;
   if (f_t) {
     const int64_t nb = fdb::adjoint_blocks(lam_dtype, f_dtype, g, lam_next, lam, f_t);

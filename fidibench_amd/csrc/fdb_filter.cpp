@@ -91,6 +91,8 @@ int fdb_filter_create(int ndims, const uint64_t* dims, const double* xmins, cons
     if (g.ntot > INT64_MAX / g.n[a]) return ffail(FDB_ERR_INVALID, "filter_create: too many cells");
     g.ntot *= g.n[a];
   }
+  if (!fdb::fits_bytes((uint64_t)g.ntot, sizeof(double)))
+    return ffail(FDB_ERR_INVALID, "filter_create: field size in bytes overflows");
   // std::map< std::vector<int>, double > order: lexicographic offsets; of
   // duplicate offsets the first inserted is kept (std::map::insert).
   std::vector<std::pair<std::vector<int>, double>> terms;

@@ -40,6 +40,21 @@ bool make_grid(int ndims, const uint64_t* n, const double* v, const double* len,
 // Bytes per stored cell for a storage dtype (FDB_F64/F32/BF16), 0 if bad.
 size_t dtype_size(int dtype);
 
+// True when ntot cells of `bytes` each fit a size_t byte count (so no
+// allocation size wraps around).
+inline bool fits_bytes(uint64_t ntot, size_t bytes) {
+  return bytes != 0 && ntot <= SIZE_MAX / bytes;
+}
+
+// Message naming a tuning variable (FDB_ROWS_CFG*, FDB_ADJ_CFG*,
+// FDB_TB_CFG_*) that does not parse or names a launch shape with no
+// instantiated kernel; nullptr when every such variable is valid.  Read
+// once per process.
+const char* tuning_error();
+
+// Key of the stochastic rounding of step `step` of a run seeded `seed`.
+uint32_t sr_step_key(uint64_t seed, uint64_t step);
+
 // Largest temporal-blocking depth the fused multi-step kernel supports.
 constexpr int kMaxTimeBlock = 8;
 
@@ -62,6 +77,10 @@ hipError_t launch_advect_pass(int dtype, const Grid3& g, void* const bufs[2], in
 // type and shape (the caller then steps one by one).
 hipError_t launch_fused_pass(int dtype, const Grid3& g, const void* src, void* dst, int k,
                              hipStream_t s, void* const* levels = nullptr);
+
+// Whether launch_fused_pass would run k fused steps of this storage type
+// (FDB_F64, FDB_F32, FDB_BF16) and shape in one launch.
+bool fused_pass_supported(int dtype, const Grid3& g, int k);
 
 // The storage layout of a dtype (FDB_BF16_SR is stored as FDB_BF16).
 inline int storage_dtype(int dtype) { return dtype == 3 ? 2 : dtype; }

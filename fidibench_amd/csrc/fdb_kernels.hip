@@ -1060,6 +1060,11 @@ struct RowsCfg {
   int R = 8, WY = 2, chunk = 0, nt = 1, grid = 0;
 };
 
+// Launch shape of the temporally blocked kernel (tb_cfg).
+struct TbCfg {
+  int R, WY, blocks;
+};
+
 constexpr int64_t kTargetBlocks = 1024;
 
 }  // namespace
@@ -1124,66 +1129,153 @@ int64_t resident_blocks(Kern kern, int threads) {
   return kernel_capacity(reinterpret_cast<const void*>(kern), threads, 0);
 }
 
-RowsCfg parse_cfg(const char* s, RowsCfg c) {
-  int r, w, ch, nt, gr = 0;
-  if (s && sscanf(s, "%d,%d,%d,%d,%d", &r, &w, &ch, &nt, &gr) >= 4) {
-    c.R = r;
-    c.WY = w;
-    c.chunk = ch;
-    c.nt = nt;
-    c.grid = gr;
+// (R, WY) pairs the rows kernel is instantiated for (FDB_ROWS_V below).
+constexpr int kRowsShapes[][2] = {{8, 2}, {4, 2}, {4, 4}, {8, 1}, {2, 4}, {2, 8}, {2, 2}, {1, 8}};
+// (R, K, WY) of the temporally blocked forward (FDB_TB below).
+constexpr int kTbShapes[][3] = {{9, 2, 4}, {9, 3, 4}, {9, 4, 4}, {9, 6, 8},
+                                {9, 7, 8}, {9, 8, 8}, {5, 8, 8}};
+
+// Default launch shape of the rows kernel.  bytes: storage of the stepped
+// field (lam for the adjoint); fbytes: the adjoint's f_t storage (0: none).
+RowsCfg rows_default(int bytes, bool adj, int fbytes) {
+  RowsCfg c;
+  // 512^3 sweeps: FP32 R=4 x 2 waves (715 G cells/s, 2.0x FP64), BF16
+  // R=2 x 4 waves.  The adjoint carries f_t as well (1024x1024x768
+  // sweeps): R=8 x 1 wave (FP32 lam and f_t, 5.3 TB/s), R=8 x 2 (FP64,
+  // 5.3 TB/s), R=4 x 4 for an FP32 lam with a bf16 f_t (4.9 TB/s; R=8
+  // spills there, and every spilled f_t load waits for its data at once).
+  if (adj) {
+    c.R = 8;
+    c.WY = 1;
+    if (bytes == 8 && fbytes == 8) c.WY = 2;
+    if (bytes == 4 && fbytes == 2) {
+      c.R = 4;
+      c.WY = 4;
+    }
+    if (bytes == 2) {  // bf16 lam: 8 cells per lane, 1 row x 8 waves (512^3 sweep)
+      c.R = 1;
+      c.WY = 8;
+    }
+  } else if (bytes == 4) {
+    c.R = 4;
+  } else if (bytes == 2) {
+    c.R = 2;
+    c.WY = 4;
   }
   return c;
 }
 
-// bytes: storage of the stepped field (lam for the adjoint); fbytes: the
-// adjoint's f_t storage (0: none).  FDB_ADJ_CFG_<bytes>_<fbytes> overrides
-// one adjoint pairing, FDB_ADJ_CFG_<bytes> all pairings of a lam type.
-RowsCfg rows_cfg(int bytes, bool adj, int fbytes = 0) {
-  static RowsCfg cfgs[2][9][9];
-  static bool init[2][9][9] = {};
-  if (!adj) fbytes = 0;
-  if (!init[adj][bytes][fbytes]) {
-    RowsCfg c;
-    // 512^3 sweeps: FP32 R=4 x 2 waves (715 G cells/s, 2.0x FP64), BF16
-    // R=2 x 4 waves.  The adjoint carries f_t as well (1024x1024x768
-    // sweeps): R=8 x 1 wave (FP32 lam and f_t, 5.3 TB/s), R=8 x 2 (FP64,
-    // 5.3 TB/s), R=4 x 4 for an FP32 lam with a bf16 f_t (4.9 TB/s; R=8
-    // spills there, and every spilled f_t load waits for its data at once).
-    if (adj) {
-      c.R = 8;
-      c.WY = 1;
-      if (bytes == 8 && fbytes == 8) c.WY = 2;
-      if (bytes == 4 && fbytes == 2) {
-        c.R = 4;
-        c.WY = 4;
-      }
-      if (bytes == 2) {  // bf16 lam: 8 cells per lane, 1 row x 8 waves (512^3 sweep)
-        c.R = 1;
-        c.WY = 8;
-      }
-    } else if (bytes == 4) {
-      c.R = 4;
-    } else if (bytes == 2) {
-      c.R = 2;
-      c.WY = 4;
-    }
-    char name[40];
-    const char* s = nullptr;
-    if (adj) {
-      snprintf(name, sizeof(name), "FDB_ADJ_CFG_%d_%d", bytes, fbytes);
-      s = getenv(name);
-      snprintf(name, sizeof(name), "FDB_ADJ_CFG_%d", bytes);
-      if (!s) s = getenv(name);
-    } else {
-      snprintf(name, sizeof(name), "FDB_ROWS_CFG_%d", bytes);
-      s = getenv(name);
-      if (!s) s = getenv("FDB_ROWS_CFG");
-    }
-    cfgs[adj][bytes][fbytes] = parse_cfg(s, c);
-    init[adj][bytes][fbytes] = true;
+TbCfg tb_default(int K) { return K >= 6 ? TbCfg{9, 8, 0} : TbCfg{9, 4, 0}; }
+
+// Launch shapes, read once per process from the environment (a
+// function-local static, so concurrent first calls are safe).  Tuning
+// variables:
+//   FDB_ROWS_CFG / FDB_ROWS_CFG_<bytes>  = "R,WY,CHUNK,NT[,GRID]"  (forward)
+//   FDB_ADJ_CFG_<bytes> / _<bytes>_<fbytes> = same                   (adjoint)
+//   FDB_TB_CFG_<bytes>_<K>               = "R,WY,BLOCKS"           (fused)
+// A variable that does not parse or names a shape without an instantiated
+// kernel makes `error` non-empty; the entry points then fail with
+// FDB_ERR_INVALID instead of launching (tuning_error()).
+struct Tuning {
+  RowsCfg rows[2][9][9];  // [adj][bytes][fbytes]
+  TbCfg tb[9][kMaxTimeBlock + 1];  // [bytes][K]
+  std::string error;
+};
+
+bool rows_shape_ok(const RowsCfg& c) {
+  if (c.chunk < 0 || c.grid < 0 || c.nt != 1) return false;  // non-temporal stores only
+  for (const auto& s : kRowsShapes)
+    if (s[0] == c.R && s[1] == c.WY) return true;
+  return false;
+}
+
+bool tb_shape_ok(const TbCfg& c, int K) {
+  if (c.blocks < 0) return false;
+  for (const auto& s : kTbShapes)
+    if (s[0] == c.R && s[1] == K && s[2] == c.WY) return true;
+  return false;
+}
+
+// Parses exactly a comma-separated list of 3 to 5 decimal integers;
+// returns how many, 0 if the text is anything else.
+int parse_ints(const char* s, int* v, int lo, int hi) {
+  int n = 0;
+  while (n < hi) {
+    char* end = nullptr;
+    const long x = strtol(s, &end, 10);
+    if (end == s || x < INT32_MIN || x > INT32_MAX) return 0;
+    v[n++] = (int)x;
+    if (*end == '\0') return n >= lo ? n : 0;
+    if (*end != ',') return 0;
+    s = end + 1;
   }
-  return cfgs[adj][bytes][fbytes];
+  return 0;
+}
+
+Tuning load_tuning() {
+  Tuning t;
+  auto bad = [&](const char* name, const char* val) {
+    if (t.error.empty())
+      t.error = std::string("tuning variable ") + name + "=\"" + val +
+                "\" does not name an instantiated launch shape";
+  };
+  auto rows_env = [&](const char* name, RowsCfg& c) {
+    const char* s = getenv(name);
+    if (!s) return false;
+    int v[5] = {0, 0, 0, 0, 0};
+    const int got = parse_ints(s, v, 4, 5);
+    RowsCfg n = c;
+    n.R = v[0];
+    n.WY = v[1];
+    n.chunk = v[2];
+    n.nt = v[3];
+    n.grid = v[4];
+    if (got == 0 || !rows_shape_ok(n)) {
+      bad(name, s);
+      return false;
+    }
+    c = n;
+    return true;
+  };
+  const int sizes[3] = {8, 4, 2};
+  char name[48];
+  for (int b : sizes) {
+    RowsCfg f = rows_default(b, false, 0);
+    snprintf(name, sizeof(name), "FDB_ROWS_CFG_%d", b);
+    if (!rows_env(name, f)) rows_env("FDB_ROWS_CFG", f);
+    for (int fb = 0; fb < 9; ++fb) t.rows[0][b][fb] = f;
+    for (int fb : {0, 8, 4, 2}) {
+      RowsCfg a = rows_default(b, true, fb);
+      snprintf(name, sizeof(name), "FDB_ADJ_CFG_%d_%d", b, fb);
+      if (!rows_env(name, a)) {
+        snprintf(name, sizeof(name), "FDB_ADJ_CFG_%d", b);
+        rows_env(name, a);
+      }
+      t.rows[1][b][fb] = a;
+    }
+    for (int K = 2; K <= kMaxTimeBlock; ++K) {
+      TbCfg c = tb_default(K);
+      snprintf(name, sizeof(name), "FDB_TB_CFG_%d_%d", b, K);
+      if (const char* s = getenv(name)) {
+        int v[3] = {0, 0, 0};
+        if (parse_ints(s, v, 3, 3) == 3 && tb_shape_ok(TbCfg{v[0], v[1], v[2]}, K))
+          c = TbCfg{v[0], v[1], v[2]};
+        else
+          bad(name, s);
+      }
+      t.tb[b][K] = c;
+    }
+  }
+  return t;
+}
+
+const Tuning& tuning() {
+  static const Tuning t = load_tuning();
+  return t;
+}
+
+RowsCfg rows_cfg(int bytes, bool adj, int fbytes = 0) {
+  return tuning().rows[adj ? 1 : 0][bytes][adj ? fbytes : 0];
 }
 
 // Launches the rows kernel for one step (forward: dirs = u, adjoint:
@@ -1274,27 +1366,12 @@ hipError_t rows_dispatch(const Grid3& g, const TS* in, TS* out, const int dirs[3
 // hipErrorNotSupported when the shape does not fit (the caller then runs
 // one-step launches): all three axes present (an absent axis has no term
 // at all, which a fused pass would not reproduce for Inf/NaN values).
-// FDB_TB_CFG_<bytes>_<K>="R,WY,BLOCKS" overrides the rows per wave, the
-// waves per block and the target number of tiles (blocks) per launch;
-// BLOCKS = 0 chooses the chunk count from the kernel's occupancy (below).
-struct TbCfg {
-  int R, WY, blocks;
-};
-
-TbCfg tb_cfg(int bytes, int K) {
-  // Sweeps at 512^3 and 1024^3 on MI355X, all bit-exact: 9 rows x 8 waves
-  // at K >= 6 (72 rows per block, 64 output rows at K=8) for FP64 and
-  // FP32; 9 rows x 4 waves at K <= 4.
-  TbCfg c{9, 4, 0};
-  if (K >= 6) c = TbCfg{9, 8, 0};
-  char name[48];
-  snprintf(name, sizeof(name), "FDB_TB_CFG_%d_%d", bytes, K);
-  if (const char* s = getenv(name)) {
-    int r, w, n;
-    if (sscanf(s, "%d,%d,%d", &r, &w, &n) == 3) c = TbCfg{r, w, n};
-  }
-  return c;
-}
+// Launch shape (rows per wave, waves per block, target tiles per launch;
+// 0 = chunk count from the kernel's occupancy, below): sweeps at 512^3
+// and 1024^3 on MI355X, all bit-exact, gave 9 rows x 8 waves at K >= 6
+// (72 rows per block, 64 output rows at K=8) for FP64 and FP32 and 9 rows
+// x 4 waves at K <= 4 (tb_default; FDB_TB_CFG_<bytes>_<K> overrides).
+TbCfg tb_cfg(int bytes, int K) { return tuning().tb[bytes][K]; }
 
 // levels != nullptr: also store levels 1..K-1 to levels[0..K-2] (K = 4
 // or 8, FP64 only: with one 4-byte cell per lane the FP32 kernel writes its
@@ -1487,6 +1564,20 @@ bool make_grid(int ndims, const uint64_t* n, const double* v, const double* len,
     g->ntot *= g->n[j];
   }
   return true;
+}
+
+bool fused_pass_supported(int dtype, const Grid3& g, int k) {
+  if (k < 2 || k > kMaxTimeBlock) return false;
+  if (dtype != FDB_F64 && dtype != FDB_F32 && dtype != FDB_BF16) return false;
+  const int bytes = (int)dtype_size(dtype);
+  const TbCfg c = tb_cfg(bytes, k);
+  return tb_shape_ok(c, k) && g.act[0] && g.act[1] && g.act[2] && c.WY * c.R > k &&
+         g.n[2] * (int64_t)bytes <= INT32_MAX;
+}
+
+const char* tuning_error() {
+  const Tuning& t = tuning();
+  return t.error.empty() ? nullptr : t.error.c_str();
 }
 
 hipError_t launch_fused_pass(int dtype, const Grid3& g, const void* src, void* dst, int k,

@@ -17,6 +17,20 @@
 // produces, the snapshot slot itself -- so storing and restoring cost no
 // copies; with enough memory for every state the forward pass moves the
 // same bytes as a plain advect.
+//
+// Shift-invariant method (FDB_GRAD_SHIFT).  The step is A = I - sum_j c_j
+// (S_j - I) with S_j the periodic unit shift along axis j and constant
+// c_j, so A, A^T and every S_j are circulant and commute.  With
+// lam_{t+1} = (A^T)^(T-1-t) lam_T and f_t = A^t f_0,
+//     <lam_{t+1}, (S_j - I) f_t> = <lam_T, A^(T-1-t) (S_j - I) A^t f_0>
+//                                = <lam_T, (S_j - I) A^(T-1) f_0>,
+// the same for every t: the velocity products of the T adjoint steps all
+// equal those of the last one (t = 0), D_j = T * <lam_1, (S_j - I) f_0>.
+// The backward is then T-1 plain transpose steps (fused K-step passes of
+// the forward kernel with the directions negated; lam_0 bit-identical to
+// the checkpointed sweep, whose lam recursion does not involve f either)
+// and one transpose step with the products against f_0: no state but f_0
+// is needed, so nothing is stored or recomputed.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -124,6 +138,9 @@ struct fdb_tape {
   bool forward_done = false;
   bool fuse_adj = false;  // this backward may use fused transpose passes
   int64_t fwd_steps = 0, adj_steps = 0;
+  int method = FDB_GRAD_CHECKPOINT;
+  bool slots_ready = false;    // snapshot slots are allocated at the first checkpointed forward
+  const void* f0_ref = nullptr;  // FDB_GRAD_SHIFT: the caller's f0
 };
 
 namespace {
@@ -356,6 +373,71 @@ int run_plan(fdb_tape* t, size_t i, bool forward_only, hipStream_t s, const void
   return FDB_OK;
 }
 
+// Grows the adjoint kernel's partial-sum scratch to nb blocks.
+int ensure_partials(fdb_tape* t, int64_t nb) {
+  const size_t pb = (size_t)nb * 6 * sizeof(double);
+  if (pb > t->partials_bytes) {
+    if (t->partials) TAPE_HIP(hipFree(t->partials));
+    t->partials = nullptr;
+    t->partials_bytes = 0;
+    TAPE_HIP(hipMalloc((void**)&t->partials, pb));
+    t->partials_bytes = pb;
+  }
+  return FDB_OK;
+}
+
+// The backward of FDB_GRAD_SHIFT: lam_T -> lam_1 by T-1 transpose steps
+// (fused passes of the forward kernel run with the directions negated,
+// same coefficients), then lam_1 -> lam_0 with the velocity products
+// against f_0, scaled by T (the products of every step are equal; see the
+// file comment).  lam_T is only read; the passes alternate between lam_tmp
+// and lam0, arranged so that lam_1 lands in lam_tmp and the last step
+// writes lam0.
+int backward_shift(fdb_tape* t, const void* lam_T, void* lam0, double* grad_v, hipStream_t s) {
+  fdb::Grid3 ga = t->g;  // the transpose step: gather from the downwind side
+  for (int j = 0; j < 3; ++j) ga.up[j] = -t->g.up[j];
+  const int ld = t->lam_dtype;
+  // Pass lengths for the T-1 plain steps: the deepest fused depth that fits.
+  std::vector<int> ks;
+  for (int left = t->nsteps - 1; left > 0;) {
+    int k = 1;
+    for (int K : {8, 7, 6, 4, 3, 2})
+      if (K <= left && K <= t->time_block && fdb::fused_pass_supported(ld, ga, K)) {
+        k = K;
+        break;
+      }
+    ks.push_back(k);
+    left -= k;
+  }
+  const void* src = lam_T;
+  for (size_t i = 0; i < ks.size(); ++i) {
+    // Counted from the end: the last pass writes lam_tmp, the one before lam0, ...
+    void* dst = ((ks.size() - 1 - i) % 2 == 0) ? t->lam_tmp : lam0;
+    if (ks[i] > 1) {
+      TAPE_HIP(fdb::launch_fused_pass(ld, ga, src, dst, ks[i], s));
+    } else {
+      TAPE_HIP(fdb::launch_adjoint_step(ld, ld, t->g, src, dst, nullptr, nullptr, nullptr, s));
+    }
+    src = dst;
+    t->adj_steps += ks[i];
+  }
+  const int64_t nb = fdb::adjoint_blocks(ld, t->f_dtype, t->g, src, lam0, t->f0_ref);
+  if (int rc = ensure_partials(t, nb)) return rc;
+  TAPE_HIP(fdb::launch_zero(t->acc, 6, s));
+  TAPE_HIP(fdb::launch_adjoint_step(ld, t->f_dtype, t->g, src, lam0, t->f0_ref, t->partials,
+                                    t->acc, s));
+  t->adj_steps += 1;
+  double acc[6];
+  TAPE_HIP(hipMemcpyAsync(acc, t->acc, sizeof(acc), hipMemcpyDeviceToHost, s));
+  TAPE_HIP(hipStreamSynchronize(s));
+  const int off = 3 - t->ndims;
+  for (int r = 0; r < t->ndims; ++r) {
+    const int j = r + off;
+    grad_v[r] = -t->g.dc[j] * ((double)t->nsteps * (acc[2 * j] + acc[2 * j + 1]));
+  }
+  return FDB_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -384,6 +466,10 @@ int fdb_tape_create(int ndims, const uint64_t* num_cells, const double* velocity
   if (fdb::dtype_size(f_dtype) == 0) return tfail(FDB_ERR_INVALID, "tape_create: f dtype");
   if (lam_dtype != FDB_F64 && lam_dtype != FDB_F32 && lam_dtype != FDB_BF16)
     return tfail(FDB_ERR_INVALID, "tape_create: lam dtype must be F64, F32 or BF16");
+  // Every byte count below (fields, the fixed working set) must not wrap.
+  if (!fdb::fits_bytes((uint64_t)g.ntot, 8 * 4))
+    return tfail(FDB_ERR_INVALID, "tape_create: field size in bytes overflows");
+  if (const char* m = fdb::tuning_error()) return tfail(FDB_ERR_INVALID, m);
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
     return tfail(FDB_ERR_NODEV, "no HIP device available");
@@ -415,9 +501,7 @@ int fdb_tape_create(int ndims, const uint64_t* num_cells, const double* velocity
   for (void*& w : t->scratch)
     if ((e = hipMalloc(&w, t->fbytes)) != hipSuccess) return fail_free(e, "hipMalloc scratch");
   if ((e = hipMalloc(&t->lam_tmp, t->lbytes)) != hipSuccess) return fail_free(e, "hipMalloc lam");
-  t->slots.resize(t->snaps, nullptr);
-  for (void*& s : t->slots)
-    if ((e = hipMalloc(&s, t->fbytes)) != hipSuccess) return fail_free(e, "hipMalloc snapshot");
+  t->slots.assign(t->snaps, nullptr);
   if ((e = hipMalloc((void**)&t->acc, 6 * sizeof(double))) != hipSuccess)
     return fail_free(e, "hipMalloc acc");
   *out = t;
@@ -426,9 +510,24 @@ int fdb_tape_create(int ndims, const uint64_t* num_cells, const double* velocity
 
 int fdb_tape_forward(fdb_tape_t* t, const void* f0, void* fT, void* stream) {
   if (!t || !f0 || !fT) return tfail(FDB_ERR_INVALID, "tape_forward: null argument");
+  if (f0 == fT) return tfail(FDB_ERR_INVALID, "tape_forward: f0 and fT must differ");
   hipStream_t s = (hipStream_t)stream;
   t->fwd_steps = 0;
   t->adj_steps = 0;
+  if (t->method == FDB_GRAD_SHIFT) {
+    // Plain forward from the caller's f0 (only read), kept for the products.
+    t->w = f0;
+    t->wstep = 0;
+    if (int rc = advance(t, t->nsteps, fT, s)) return rc;
+    t->f0_ref = f0;
+    TAPE_HIP(hipStreamSynchronize(s));
+    t->forward_done = true;
+    return FDB_OK;
+  }
+  if (!t->slots_ready) {
+    for (void*& sl : t->slots) TAPE_HIP(hipMalloc(&sl, t->fbytes));
+    t->slots_ready = true;
+  }
   TAPE_HIP(hipMemcpyAsync(t->base, f0, t->fbytes, hipMemcpyDeviceToDevice, s));
   t->w = t->base;
   t->wstep = 0;
@@ -456,6 +555,7 @@ int fdb_tape_backward(fdb_tape_t* t, const void* lam_T, void* lam0, double* grad
   if (lam_T == lam0) return tfail(FDB_ERR_INVALID, "tape_backward: lam_T and lam0 must differ");
   if (!t->forward_done) return tfail(FDB_ERR_INVALID, "tape_backward: run tape_forward first");
   hipStream_t s = (hipStream_t)stream;
+  if (t->method == FDB_GRAD_SHIFT) return backward_shift(t, lam_T, lam0, grad_v, s);
   // Partial-sum scratch of the adjoint kernel, sized for every buffer
   // pairing the sweep uses (f_t is always a tape buffer).
   int64_t nb = 0;
@@ -471,14 +571,7 @@ int fdb_tape_backward(fdb_tape_t* t, const void* lam_T, void* lam0, double* grad
     t->fuse_adj = fb > 0;
     nb = std::max(nb, fb);
   }
-  const size_t pb = (size_t)nb * 6 * sizeof(double);
-  if (pb > t->partials_bytes) {
-    if (t->partials) TAPE_HIP(hipFree(t->partials));
-    t->partials = nullptr;
-    t->partials_bytes = 0;
-    TAPE_HIP(hipMalloc((void**)&t->partials, pb));
-    t->partials_bytes = pb;
-  }
+  if (int rc = ensure_partials(t, nb)) return rc;
   TAPE_HIP(fdb::launch_zero(t->acc, 6, s));
   const void* lam_next = lam_T;
   int done = 0;
@@ -507,10 +600,18 @@ int fdb_tape_set_time_block(fdb_tape_t* t, int k) {
   return FDB_OK;
 }
 
+int fdb_tape_set_method(fdb_tape_t* t, int method) {
+  if (!t || (method != FDB_GRAD_CHECKPOINT && method != FDB_GRAD_SHIFT))
+    return tfail(FDB_ERR_INVALID, "tape_set_method: unknown method");
+  if (method != t->method) t->forward_done = false;  // a backward needs this method's forward
+  t->method = method;
+  return FDB_OK;
+}
+
 int fdb_tape_stats(const fdb_tape_t* t, int* snaps, int64_t* forward_steps,
                    int64_t* adjoint_steps) {
   if (!t) return tfail(FDB_ERR_INVALID, "tape_stats: null tape");
-  if (snaps) *snaps = t->snaps;
+  if (snaps) *snaps = t->method == FDB_GRAD_SHIFT ? 0 : t->snaps;
   if (forward_steps) *forward_steps = t->fwd_steps;
   if (adjoint_steps) *adjoint_steps = t->adj_steps;
   return FDB_OK;

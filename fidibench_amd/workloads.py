@@ -72,14 +72,16 @@ def compare(a: torch.Tensor, b: torch.Tensor) -> dict:
 
 
 def misfit_gradient(w_eval: Workload, target: torch.Tensor, f_dtype: int, lam_dtype: int,
-                    max_bytes: int, f0: torch.Tensor | None = None) -> dict:
-    """Checkpointed gradient of the misfit at w_eval's velocity and field."""
+                    max_bytes: int, f0: torch.Tensor | None = None,
+                    method: str = "checkpoint") -> dict:
+    """Gradient of the misfit at w_eval's velocity and field, by the tape's
+    checkpointed or shift-invariant method."""
     if f0 is None:
         f0 = init_field(w_eval, f_dtype)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     tape = Tape(w_eval.num_cells, w_eval.velocity, w_eval.lengths, w_eval.dt, w_eval.steps,
-                f_dtype, lam_dtype, max_bytes)
+                f_dtype, lam_dtype, max_bytes, method=method)
     t1 = time.perf_counter()
     fT = torch.empty_like(f0)
     tape.forward(f0, fT)

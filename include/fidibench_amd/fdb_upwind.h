@@ -147,6 +147,17 @@ int fdb_advect_dev(int ndims, const uint64_t* num_cells, const double* velocity,
                    const double* lengths, double delta_time, int dtype, void* a, void* b,
                    int nsteps, int time_block, void* stream, int* result_in_b);
 
+/* fdb_advect_dev with the stochastic-rounding stream made explicit: for
+ * FDB_BF16_SR the random bits of the i-th step of this call are keyed by
+ * (seed, step0 + i), as a handle seeded `seed` keys its step step0 + i, so
+ * calls with step0 = 0, n, 2n, ... continue one run (fdb_advect_dev is
+ * seed 0, step0 0).  Other dtypes ignore both.  Extension (no counterpart
+ * in the reference). */
+int fdb_advect_dev_seeded(int ndims, const uint64_t* num_cells, const double* velocity,
+                          const double* lengths, double delta_time, int dtype, void* a, void* b,
+                          int nsteps, int time_block, uint64_t seed, uint64_t step0,
+                          void* stream, int* result_in_b);
+
 /* One adjoint step (the exact transpose of one forward step, gathering
  * from the downwind side with the same coefficients): lam = A^T lam_next.
  * If f_t is non-null, also accumulates into vgrad_acc (device, 6 doubles:
@@ -191,6 +202,22 @@ int fdb_tape_backward(fdb_tape_t* t, const void* lam_T, void* lam0, double* grad
  * every k; dL/dv sums the same products in another order (equal to
  * rounding).  Extension (no counterpart in the reference). */
 int fdb_tape_set_time_block(fdb_tape_t* t, int k);
+/* How fdb_tape_backward forms the gradient (extension; the reference has
+ * no gradient):
+ *   FDB_GRAD_CHECKPOINT  (default) every adjoint step with its velocity
+ *       products against the forward state of that step, the states
+ *       recomputed from binomial checkpoints within max_bytes;
+ *   FDB_GRAD_SHIFT  the step and the periodic axis shifts are circulant
+ *       and commute, so the products of every step are equal: lam_0 by
+ *       T transpose steps (fused passes, bit-identical to the checkpointed
+ *       lam_0) and dL/dv = -(dt*u_j/dx_j) * T * <lam_1, S_j f0 - f0>
+ *       (equal to the per-step sum up to rounding).  No state is stored:
+ *       the tape keeps a reference to the caller's f0, which must stay
+ *       unchanged until the last fdb_tape_backward; backward may be called
+ *       again (another lam_T) after one forward.
+ * Set before fdb_tape_forward; changing it requires a new forward. */
+enum fdb_grad_method { FDB_GRAD_CHECKPOINT = 0, FDB_GRAD_SHIFT = 1 };
+int fdb_tape_set_method(fdb_tape_t* t, int method);
 /* Checkpoint slots used, forward steps and adjoint steps executed so far. */
 int fdb_tape_stats(const fdb_tape_t* t, int* snaps, int64_t* forward_steps,
                    int64_t* adjoint_steps);

@@ -7,6 +7,7 @@ import ctypes as C
 import os
 import re
 import subprocess
+import sys
 from math import comb
 
 import numpy as np
@@ -210,7 +211,9 @@ def test_create_without_gpu_is_nodev():
 
 def test_fp64_kernels_have_no_fused_multiply_add(tmp_path):
     """-ffp-contract=off must hold: no v_fma_f64 / v_fmac_f64 in the FP64
-    forward kernels, and no packed FP32 instructions anywhere."""
+    one-step kernels (forward and adjoint) nor in any FP64 instance of the
+    temporally blocked kernel (plain, store-all and fused-adjoint modes),
+    and no packed FP32 instructions anywhere."""
     src = os.path.join(ROOT, "fidibench_amd", "csrc", "fdb_kernels.hip")
     asm = tmp_path / "k.s"
     subprocess.run(["/opt/rocm/bin/hipcc", "--offload-arch=gfx950", "-O3", "-std=c++17",
@@ -218,10 +221,76 @@ def test_fp64_kernels_have_no_fused_multiply_add(tmp_path):
                     "-packed-fp32-ops", "--cuda-device-only", "-S", "-o", str(asm), src],
                    check=True, capture_output=True)
     text = asm.read_text()
-    bodies = re.findall(r"^(_ZN3fdb\S*(?:rows|march)_kernelId\S*):[^\n]*\n(.*?)s_endpgm", text,
-                        re.M | re.S)
-    fwd = [(n, b) for n, b in bodies if "Lb0E" in n or "rows_kernel" in n]
-    assert len(fwd) >= 4
-    for name, body in fwd:
+    bodies = re.findall(r"^(_ZN3fdb\S*(?:rows|march|tb)_kernelId\S*):[^\n]*\n(.*?)s_endpgm",
+                        text, re.M | re.S)
+    rows = [n for n, _ in bodies if "rows_kernel" in n]
+    fused = [n for n, _ in bodies if "tb_kernel" in n]
+    assert len(rows) >= 4 and len(fused) >= 30, (len(rows), len(fused))
+    for name, body in bodies:
         assert not re.search(r"v_fmac?_f64", body), name
     assert not re.search(r"v_pk_(add|mul|fma)_f32", text)
+
+
+# ------------------------------------------------------- tuning variables
+
+_CREATE = r"""
+import ctypes as C, sys
+sys.path.insert(0, %(root)r)
+from fidibench_amd import _lib as L
+h = C.c_void_p()
+rc = L.lib().fdb_upwind_create(3, L.dbls([1] * 3), L.dbls([1] * 3), L.u64s([8] * 3), 0,
+                               C.byref(h))
+print(rc, L.lib().fdb_last_error().decode())
+if rc == 0:
+    L.lib().fdb_upwind_destroy(h)
+"""
+
+
+def _create_with_env(extra):
+    env = dict(os.environ, **extra)
+    out = subprocess.run([sys.executable, "-c", _CREATE % {"root": ROOT}], env=env,
+                         capture_output=True, text=True, timeout=300, check=True)
+    rc, _, msg = out.stdout.strip().splitlines()[-1].partition(" ")
+    return int(rc), msg
+
+
+@pytest.mark.parametrize("var,val", [
+    ("FDB_TB_CFG_8_6", "7,3,0"),        # no (7 rows, K=6, 3 waves) instance
+    ("FDB_TB_CFG_4_8", "9,8"),           # too few fields
+    ("FDB_ROWS_CFG_8", "16,2,0,1"),      # no 16-row instance
+    ("FDB_ROWS_CFG", "8,2,0,0"),         # cached (non-nontemporal) stores: not instantiated
+    ("FDB_ADJ_CFG_4_2", "4,4,0,1,x"),    # trailing garbage
+])
+def test_bad_tuning_variable_is_invalid_argument(var, val):
+    """A tuning variable naming a launch shape without a kernel is an
+    argument error, reported before any device work (no GPU needed)."""
+    L = _lib()
+    rc, msg = _create_with_env({var: val})
+    assert rc == L.ERR_INVALID and var in msg, (rc, msg)
+
+
+def test_valid_tuning_variable_is_accepted():
+    L = _lib()
+    rc, msg = _create_with_env({"FDB_TB_CFG_8_6": "9,8,1024", "FDB_TB_CFG_8_4": "9,4,0",
+                                "FDB_ROWS_CFG_4": "8,1,0,1"})
+    assert rc in (L.OK, L.ERR_NODEV), (rc, msg)
+
+
+def test_byte_count_overflow_is_invalid_argument():
+    """ntot * itemsize beyond size_t is rejected before any allocation
+    (2^62 cells of 8 bytes would wrap to 0 bytes)."""
+    L = _lib()
+    lib = L.lib()
+    n = [1 << 21, 1 << 21, 1 << 20]
+    h = C.c_void_p()
+    assert lib.fdb_upwind_create(3, L.dbls([1] * 3), L.dbls([1] * 3), L.u64s(n), L.F64,
+                                 C.byref(h)) == L.ERR_INVALID
+    assert b"overflows" in lib.fdb_last_error()
+    assert lib.fdb_tape_create(3, L.u64s(n), L.dbls([1] * 3), L.dbls([1] * 3), 1e-3, 4, L.F64,
+                               L.F64, 1 << 30, C.byref(h)) == L.ERR_INVALID
+    assert b"overflows" in lib.fdb_last_error()
+    offs = (C.c_int * 3)(0, 0, 0)
+    w = (C.c_double * 1)(1.0)
+    assert lib.fdb_filter_create(3, L.u64s([(1 << 31) - 1, (1 << 31) - 1, 1]), L.dbls([0] * 3),
+                                 L.dbls([1] * 3), 1, offs, w, C.byref(h)) == L.ERR_INVALID
+    assert b"overflows" in lib.fdb_last_error()

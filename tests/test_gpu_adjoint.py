@@ -159,11 +159,12 @@ def test_dot_product_identity_on_gpu():
     assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
 
 
-def _tape_grad(f0, target, v, dt, T, max_bytes=None, f_dtype=torch.float64):
+def _tape_grad(f0, target, v, dt, T, max_bytes=None, f_dtype=torch.float64,
+               method="checkpoint"):
     from fidibench_amd.autograd import advect, misfit
     f0t = torch.from_numpy(f0).cuda().to(f_dtype).requires_grad_(True)
     vt = torch.tensor(v, dtype=torch.float64, requires_grad=True)
-    fT = advect(f0t, vt, dt, T, max_bytes=max_bytes)
+    fT = advect(f0t, vt, dt, T, max_bytes=max_bytes, method=method)
     loss = misfit(fT, torch.from_numpy(target).cuda().to(f_dtype))
     loss.backward()
     return float(loss.detach()), f0t.grad.double().cpu().numpy(), vt.grad.numpy().copy(), fT
@@ -171,20 +172,63 @@ def _tape_grad(f0, target, v, dt, T, max_bytes=None, f_dtype=torch.float64):
 
 # 3-D cases run the fused store-all forward and fused adjoint passes (T = 11
 # leaves 10 stored states); 2-D and 1-D grids take the one-step kernels.
+# The shift-invariant method forms dL/dv from one step's products times T
+# instead of the sum over steps (the oracle's order): equal up to rounding.
+@pytest.mark.parametrize("method", ["checkpoint", "shift"])
 @pytest.mark.parametrize("shape,v,T", [((10, 12, 70), (0.45, -0.3, 0.2), 6),
                                        ((12, 40, 100), (-0.4, 0.3, -0.25), 11),
                                        ((24, 70), (0.5, -0.35), 9),
-                                       ((130,), (-0.6,), 9)])
-def test_gradient_matches_oracle(oracle, shape, v, T):
+                                       ((130,), (-0.6,), 9),
+                                       ((5, 9, 17), (-0.45, 0.3, -0.2), 1)])
+def test_gradient_matches_oracle(oracle, shape, v, T, method):
     shape, v, dt, T, f0, target = _setup(shape=shape, v=v, T=T)
     lengths = (1.0,) * len(shape)
-    loss, gf, gv, fT = _tape_grad(f0, target, v, dt, T)
+    loss, gf, gv, fT = _tape_grad(f0, target, v, dt, T, method=method)
     rl, rgf, rgv = oracle.misfit_grad(f0, target, v, lengths, dt, T)
     assert np.array_equal(bits(fT.detach().cpu().numpy()),
                           bits(oracle.advect(f0, v, lengths, dt, T)))
     assert abs(loss - rl) <= 1e-14 * rl
     assert np.array_equal(bits(gf), bits(rgf))  # same adjoint op order: bit-exact
-    assert np.allclose(gv, rgv, rtol=1e-12, atol=0)
+    assert np.allclose(gv, rgv, rtol=1e-12 if method == "checkpoint" else 1e-11, atol=0), \
+        (gv, rgv)
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+@pytest.mark.parametrize("shape,v,T", [((20, 70, 130), (0.45, -0.3, 0.2), 37),
+                                       ((33, 45, 200), (-0.6, 0.2, -0.3), 20),
+                                       ((24, 70), (0.5, -0.35), 13)])
+def test_shift_method_equals_checkpointed_sweep(dtype, shape, v, T):
+    """The shift-invariant backward against the per-step checkpointed one
+    on the same forward: lam_0 bit for bit (neither lam recursion involves
+    f), dL/dv equal up to rounding (FP64) or to the storage's precision
+    (FP32 states differ from exact by ~1e-7 relative); fT identical; no
+    snapshots; a second backward from the same forward gives the same."""
+    from fidibench_amd import _lib as L
+    from fidibench_amd.autograd import Tape
+    fd, tdt = (L.F64, torch.float64) if dtype == "f64" else (L.F32, torch.float32)
+    g = torch.Generator().manual_seed(T)
+    f0 = torch.rand(shape, generator=g, dtype=torch.float64).to(tdt).cuda()
+    lamT = torch.randn(shape, generator=g, dtype=torch.float64).to(tdt).cuda()
+    dt = 0.5 / max(shape) / sum(abs(x) for x in v)
+    out = {}
+    for method in ("checkpoint", "shift"):
+        tape = Tape(shape, v, (1,) * len(shape), dt, T, fd, fd, 1 << 31, method=method)
+        fT = torch.empty_like(f0)
+        tape.forward(f0, fT)
+        lam0 = torch.empty_like(lamT)
+        gv = np.array(tape.backward(lamT, lam0))
+        st = tape.stats()
+        if method == "shift":
+            assert st == {"snapshots": 0, "forward_steps": T, "adjoint_steps": T}, st
+            lam0b = torch.empty_like(lamT)
+            assert np.array_equal(np.array(tape.backward(lamT, lam0b)), gv)
+            assert torch.equal(lam0b.view(torch.uint8), lam0.view(torch.uint8))
+        tape.close()
+        out[method] = (fT.cpu(), lam0.cpu(), gv)
+    (fc, lc, gc), (fs, ls, gs) = out["checkpoint"], out["shift"]
+    assert torch.equal(fc.view(torch.uint8), fs.view(torch.uint8))
+    assert torch.equal(lc.view(torch.uint8), ls.view(torch.uint8))
+    assert np.allclose(gs, gc, rtol=1e-11 if dtype == "f64" else 2e-5, atol=0), (gs, gc)
 
 
 @pytest.mark.parametrize("snaps", [0, 1, 3])
@@ -196,7 +240,8 @@ def test_checkpointing_is_exact(snaps):
     full = _tape_grad(f0, target, v, dt, T)
     fbytes = int(np.prod(shape)) * 8
     # fixed working set: initial state, two working buffers, one adjoint buffer
-    part = _tape_grad(f0, target, v, dt, T, max_bytes=(4 + snaps) * fbytes)
+    part = _tape_grad(f0, target, v, dt, T, max_bytes=(4 + snaps) * fbytes,
+                      method="checkpoint")
     assert np.array_equal(bits(full[1]), bits(part[1]))
     assert np.allclose(full[2], part[2], rtol=1e-13, atol=0)
 
@@ -338,12 +383,13 @@ def test_tape_forward_steps_follow_binomial_optimum():
         tape.close()
 
 
-def test_gradient_vs_central_differences(oracle):
+@pytest.mark.parametrize("method", ["checkpoint", "shift"])
+def test_gradient_vs_central_differences(oracle, method):
     """FP64: <grad, d> against (L(x+eps d) - L(x-eps d)) / (2 eps), eps = 1e-6,
     on 20 seeded random directions in (f0, v)."""
     from fidibench_amd.autograd import advect, misfit
     shape, v, dt, T, f0, target = _setup(shape=(16, 16, 64), T=12, seed=9)
-    loss, gf, gv, _ = _tape_grad(f0, target, v, dt, T)
+    loss, gf, gv, _ = _tape_grad(f0, target, v, dt, T, method=method)
     f0t = torch.from_numpy(f0).cuda()
     tg = torch.from_numpy(target).cuda()
     vt = torch.tensor(v, dtype=torch.float64)
@@ -366,9 +412,10 @@ def test_gradient_vs_central_differences(oracle):
     assert worst < 1e-6, worst
 
 
-def test_gradient_vs_tlm(oracle):
+@pytest.mark.parametrize("method", ["checkpoint", "shift"])
+def test_gradient_vs_tlm(oracle, method):
     shape, v, dt, T, f0, target = _setup(shape=(9, 11, 40), T=8, seed=13)
-    loss, gf, gv, fT = _tape_grad(f0, target, v, dt, T)
+    loss, gf, gv, fT = _tape_grad(f0, target, v, dt, T, method=method)
     rng = np.random.default_rng(7)
     fTn = fT.detach().cpu().numpy()
     for _ in range(5):
@@ -380,10 +427,11 @@ def test_gradient_vs_tlm(oracle):
         assert abs(lhs - rhs) <= 1e-11 * abs(lhs)
 
 
-def test_fp32_adjoint_close_to_fp64():
+@pytest.mark.parametrize("method", ["checkpoint", "shift"])
+def test_fp32_adjoint_close_to_fp64(method):
     shape, v, dt, T, f0, target = _setup(shape=(16, 16, 64), T=10, seed=17)
-    l64, g64, v64, _ = _tape_grad(f0, target, v, dt, T)
-    l32, g32, v32, _ = _tape_grad(f0, target, v, dt, T, f_dtype=torch.float32)
+    l64, g64, v64, _ = _tape_grad(f0, target, v, dt, T, method=method)
+    l32, g32, v32, _ = _tape_grad(f0, target, v, dt, T, f_dtype=torch.float32, method=method)
     assert np.abs(g32 - g64).max() <= 1e-5 * np.abs(g64).max()
     assert np.allclose(v32, v64, rtol=1e-4)
 

@@ -1,0 +1,112 @@
+"""The mixed-precision and large-gradient workloads at their full sizes.
+
+gauss1024: 1024^3 cells, 200 steps, v = (0.7, -0.4, 0.2), CFL 0.5, Gaussian
+bubble exp(-|x-c|^2/0.005) at a seeded centre.  FP64 (bit-exact with the
+reference operation order; here it is the yardstick) against FP32, bf16
+storage rounded to nearest even and bf16 with stochastic rounding: max-abs,
+RMS and max-rel error against the FP64 field and relative mass drift
+(compensated FP64 device sums), each against the bound stated below.
+FP64 fused 8 steps per pass must equal one step per launch bit for bit.
+
+misfit_large: 1024 x 1024 x 768 cells, 400 steps, 8 Gaussian bubbles
+a*exp(-|x-c|^2/w^2); the gradient of 0.5*sum((f_T - target)^2) with a
+bf16 stochastic-rounding forward and an FP32 adjoint, by the checkpointed
+sweep (snapshots sized to the free HBM, optimal binomial schedule) and by
+the shift-invariant method, against the FP64 gradient.
+"""
+from __future__ import annotations
+
+from math import comb
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+# Stated bounds for gauss1024 (measured: FP32 max-rel 4.0e-7, drift 7.8e-10;
+# bf16 RNE max-abs 0.378, RMS 6.8e-3, drift 0.168 -- round to nearest drops
+# every increment below half an ulp, so smooth regions stagnate; bf16 SR
+# max-abs 0.0106, RMS 8.8e-5, drift 8.7e-6).
+BOUNDS = {
+    "f32": {"max_rel": 1e-5, "rms": 1e-7, "drift": 1e-8},
+    "bf16": {"max_abs": 0.5, "rms": 1e-2, "drift": 0.25},
+    "bf16sr": {"max_abs": 0.02, "rms": 2e-4, "drift": 5e-5},
+}
+
+
+def _compare(a, b):
+    import ctypes as C
+    from fidibench_amd import _lib as L
+    out = (C.c_double * 4)()
+    L.check(L.lib().fdb_compare_dev(a.dtype, a.device_ptr(), b.dtype, b.device_ptr(), a.ntot,
+                                    out, None))
+    return {"max_abs": out[0], "rms": out[1], "max_rel": out[3]}
+
+
+def test_mixed_precision_gauss1024():
+    from fidibench_amd import Upwind, configs as K
+    w = K.gauss1024()
+    ups, drift = {}, {}
+    for name, tb in (("f64", 8), ("f64_k1", 1), ("f32", 8), ("bf16", 8), ("bf16sr", 8)):
+        up = Upwind(w.velocity, w.lengths, w.num_cells, dtype=name.split("_")[0])
+        up.set_time_block(tb)
+        up.init_field(w.init_kind, w.init_params, seed=w.init_seed)
+        m0 = up.mass()
+        up.advect(w.steps, w.dt)
+        drift[name] = abs(up.mass() - m0) / abs(m0)
+        ups[name] = up
+    same = _compare(ups["f64_k1"], ups["f64"])
+    assert same["max_abs"] == 0.0 and ups["f64_k1"].mass() == ups["f64"].mass()
+    assert drift["f64"] < 1e-13, drift
+    errs = {n: _compare(ups[n], ups["f64"]) for n in ("f32", "bf16", "bf16sr")}
+    print("gauss1024 errors vs FP64", errs, "drift", drift)
+    for n, b in BOUNDS.items():
+        for key, lim in b.items():
+            got = drift[n] if key == "drift" else errs[n][key]
+            assert got <= lim, (n, key, got, lim)
+    # stochastic rounding is the accurate bf16 variant
+    assert errs["bf16sr"]["rms"] < 0.05 * errs["bf16"]["rms"]
+    assert drift["bf16sr"] < 1e-3 * drift["bf16"]
+    for up in ups.values():
+        up.close()
+
+
+def test_large_gradient_bf16_forward_fp32_adjoint():
+    from fidibench_amd import _lib as L, configs as K
+    from fidibench_amd.workloads import advect, compare, init_field, misfit_gradient
+    w = K.misfit_large()
+    wt = K.Workload("misfit_large_target", w.num_cells, w.steps, (0.8, 0.3, -0.5), w.init_kind,
+                    w.init_params, dt_velocity=w.dt_velocity)
+    target = advect(init_field(wt, L.F64), wt.velocity, wt, wt.steps)
+    g64 = misfit_gradient(w, target, L.F64, L.F64, 16 << 30, method="shift")
+    lam64, gv64 = g64.pop("grad_f0"), np.array(g64.pop("grad_v"))
+    fbytes = w.ntot * 2
+    free = torch.cuda.mem_get_info()[0]
+    budget = int(free * 0.8)
+    res = {}
+    for method in ("checkpoint", "shift"):
+        g = misfit_gradient(w, target, L.BF16_SR, L.F32, budget, method=method)
+        lam, gv = g.pop("grad_f0"), np.array(g.pop("grad_v"))
+        res[method] = (lam, gv, g)
+    lc, gvc, stc = res["checkpoint"]
+    ls, gvs, sts = res["shift"]
+    # the checkpointed sweep: snapshots within the budget, binomial optimum
+    s = stc["snapshots"]
+    assert 0 < s < w.steps - 1 and (3 + s) * fbytes + w.ntot * 4 <= budget, stc
+    r = 0
+    while comb(s + 1 + r, s + 1) < w.steps:
+        r += 1
+    assert stc["forward_steps"] == r * w.steps - comb(s + 1 + r, s + 2) + 1, stc
+    assert sts["snapshots"] == 0 and sts["forward_steps"] == w.steps
+    # same forward, same lam recursion: dL/df0 identical for both methods
+    assert torch.equal(lc.view(torch.int32), ls.view(torch.int32))
+    err = compare(lc, lam64)
+    rel_c = np.abs(gvc - gv64) / np.abs(gv64)
+    rel_s = np.abs(gvs - gv64) / np.abs(gv64)
+    print("misfit_large bf16/fp32 vs fp64: dL/df0", err, "dL/dv rel (checkpoint)", rel_c,
+          "(shift)", rel_s, "loss", stc["loss"], g64["loss"])
+    assert abs(stc["loss"] - g64["loss"]) <= 1e-3 * g64["loss"]
+    assert err["max_rel"] < 1e-2 and err["rms"] < 2e-4 * float(lam64.abs().max())
+    assert rel_c.max() < 1e-3 and rel_s.max() < 1e-3

@@ -2,15 +2,22 @@
 
 Target: the sphere advected 50 steps at v* = (0.8, 0.3, -0.5).  Gradient
 at v = (0.9, 0.25, -0.45) from the sphere + 0.01 noise, dt fixed by
-sum|v*| = 1.6.  FP64: checked by centred finite differences of the GPU
-forward (eps = 1e-6) along 20 seeded random directions in (f0, v), within
-1e-6 relative.  FP32: stated against FP64.  The FP64 CPU tangent-linear
-check runs at reduced size in test_gpu_adjoint.py.
+sum|v*| = 1.6.  FP64, both gradient methods (checkpointed per-step sweep
+and shift-invariant): checked by centred finite differences of the GPU
+forward (eps = 1e-6) along 20 seeded random directions in (f0, v), and
+against the oracle's FP64 tangent-linear model at this full size (5
+directions, tests/golden/tlm256.json from tests/golden/gen_tlm256.py),
+within 1e-6 relative.  FP32: stated against FP64.
 """
 from __future__ import annotations
 
+import json
+import os
+
 import numpy as np
 import pytest
+
+from conftest import GOLDEN
 
 pytestmark = pytest.mark.gpu
 
@@ -26,11 +33,12 @@ def setup():
     return w, target
 
 
-def test_gradient_fd_20_directions(setup):
+@pytest.mark.parametrize("method", ["checkpoint", "shift"])
+def test_gradient_fd_20_directions(setup, method):
     from fidibench_amd import _lib as L
     from fidibench_amd.workloads import advect, init_field, misfit, misfit_gradient
     w, target = setup
-    g = misfit_gradient(w, target, L.F64, L.F64, 40 << 30)
+    g = misfit_gradient(w, target, L.F64, L.F64, 40 << 30, method=method)
     assert g["adjoint_steps"] == w.steps
     f0 = init_field(w, L.F64)
     v = np.array(w.velocity)
@@ -49,12 +57,41 @@ def test_gradient_fd_20_directions(setup):
     assert worst < 1e-6
 
 
-def test_fp32_gradient_vs_fp64(setup):
+@pytest.mark.parametrize("method", ["checkpoint", "shift"])
+def test_gradient_vs_tlm_fixture(setup, method):
+    """<dL/df0, df0> + <dL/dv, dv> against the oracle's FP64 tangent-linear
+    directional derivatives <f_T - target, df_T> at 256^3 x 50, the
+    directions rebuilt on the GPU from their SplitMix64 seeds (their exact
+    sums are in the fixture)."""
+    from fidibench_amd import _lib as L
+    from fidibench_amd.autograd import compensated_sum
+    from fidibench_amd.workloads import misfit_gradient
+    w, target = setup
+    with open(os.path.join(GOLDEN, "tlm256.json")) as fh:
+        fx = json.load(fh)
+    assert float.fromhex(float(fx["dt"]).hex()) == w.dt
+    g = misfit_gradient(w, target, L.F64, L.F64, 40 << 30, method=method)
+    assert abs(g["loss"] - fx["loss"]) <= 1e-12 * fx["loss"], (g["loss"], fx["loss"])
+    for d in fx["directions"]:
+        df0 = torch.empty(w.num_cells, dtype=torch.float64, device="cuda")
+        L.check(L.lib().fdb_init_field_dev(3, L.u64s(w.num_cells), L.dbls(w.lengths), L.F64,
+                                           L.INIT_UNIFORM, L.dbls(d["df0_range"]), 2,
+                                           d["df0_seed"], df0.data_ptr(),
+                                           torch.cuda.current_stream().cuda_stream))
+        assert abs(compensated_sum(df0) - d["df0_sum"]) <= 1e-9 * max(1.0, abs(d["df0_sum"]))
+        an = compensated_sum(g["grad_f0"] * df0) + float(np.dot(g["grad_v"], d["dv"]))
+        rel = abs(an - d["dLdelta"]) / abs(d["dLdelta"])
+        print(method, d["df0_seed"], an, d["dLdelta"], rel)
+        assert rel < 1e-6, (an, d["dLdelta"])
+
+
+@pytest.mark.parametrize("method", ["checkpoint", "shift"])
+def test_fp32_gradient_vs_fp64(setup, method):
     from fidibench_amd import _lib as L
     from fidibench_amd.workloads import compare, misfit_gradient
     w, target = setup
-    g64 = misfit_gradient(w, target, L.F64, L.F64, 40 << 30)
-    g32 = misfit_gradient(w, target, L.F32, L.F32, 40 << 30)
+    g64 = misfit_gradient(w, target, L.F64, L.F64, 40 << 30, method=method)
+    g32 = misfit_gradient(w, target, L.F32, L.F32, 40 << 30, method=method)
     err = compare(g32["grad_f0"], g64["grad_f0"])
     rel_v = np.abs(np.array(g32["grad_v"]) - g64["grad_v"]) / np.abs(g64["grad_v"])
     print("fp32 dL/df0 error", err, "dL/dv rel error", rel_v)

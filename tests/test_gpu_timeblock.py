@@ -111,3 +111,66 @@ def test_two_plane_march_both_parities(oracle, monkeypatch, dtype, k, n0, blocks
     assert launches < 2 * k + 1
     assert np.array_equal(bits(up.get_field()), bits(ref)), (dtype, k, n0, blocks)
     up.close()
+
+
+# Odd row lengths: bf16 rows then start on 2-byte (and FP32 rows on 4-byte)
+# boundaries inside the fused kernel's raw buffer accesses; n2 = 1 and 3
+# make every lane of a tile wrap onto the same few columns.
+ODD = [(5, 48, 67), (6, 17, 131), (4, 21, 3), (7, 9, 1)]
+
+
+@pytest.mark.parametrize("dtype", ["f32", "bf16"])
+@pytest.mark.parametrize("shape", ODD)
+@pytest.mark.parametrize("signs", [(1, 1, 1), (-1, 1, -1), (1, -1, 1), (-1, -1, -1)])
+def test_time_blocked_odd_rows_bit_exact(oracle, dtype, shape, signs):
+    from fidibench_amd import Upwind, configs as K
+    v = tuple(s * m for s, m in zip(signs, (0.3, 0.45, 0.2)))
+    dt = K.cfl_dt(shape, v)
+    f0 = oracle.uniform(shape, 0.0, 1.0, seed=sum(shape) + 11)
+    for k in (2, 3, 4, 6, 7, 8):
+        up = Upwind(v, (1, 1, 1), shape, dtype=dtype)
+        up.set_field(f0)
+        start = up.get_field()
+        ref = oracle.advect(start, v, (1, 1, 1), dt, 11)
+        up.set_time_block(k)
+        ms, launches = up.advect_timed(11, dt)
+        assert np.array_equal(bits(up.get_field()), bits(ref)), (dtype, shape, signs, k)
+        assert launches < 11
+        up.close()
+
+
+@pytest.mark.parametrize("time_block", [1, 8])
+def test_device_pointer_bf16_sr_continues_one_run(oracle, time_block):
+    """fdb_advect_dev_seeded with FDB_BF16_SR keys step i of a call by
+    (seed, step0 + i): two calls of 4 and 5 steps equal one 9-step oracle
+    run with that seed, bit for bit, for any time_block."""
+    import ctypes as C
+    import torch
+    from fidibench_amd import _lib as L, configs as K
+    shape, v, seed = (9, 20, 70), (0.4, -0.3, 0.25), 987654321
+    dt = K.cfl_dt(shape, v)
+    f0 = oracle.f32_to_bf16(oracle.uniform(shape, 0.0, 1.0, seed=4)).reshape(shape)
+    ref = oracle.advect_bf16_sr(f0, v, (1, 1, 1), dt, 9, seed=seed, step0=0)
+    a = torch.from_numpy(f0.view(np.int16).copy()).cuda()
+    b = torch.empty_like(a)
+    res = C.c_int()
+    s = torch.cuda.current_stream().cuda_stream
+    step0 = 0
+    for n in (4, 5):
+        L.check(L.lib().fdb_advect_dev_seeded(3, L.u64s(shape), L.dbls(v), L.dbls((1, 1, 1)), dt,
+                                              L.BF16_SR, a.data_ptr(), b.data_ptr(), n,
+                                              time_block, seed, step0, s, C.byref(res)))
+        if res.value:
+            a, b = b, a
+        step0 += n
+    torch.cuda.synchronize()
+    got = a.cpu().numpy().view(np.uint16)
+    assert np.array_equal(got, ref)
+    # the unseeded entry point is seed 0, step 0
+    ref0 = oracle.advect_bf16_sr(f0, v, (1, 1, 1), dt, 3, seed=0, step0=0)
+    a = torch.from_numpy(f0.view(np.int16).copy()).cuda()
+    L.check(L.lib().fdb_advect_dev(3, L.u64s(shape), L.dbls(v), L.dbls((1, 1, 1)), dt, L.BF16_SR,
+                                   a.data_ptr(), b.data_ptr(), 3, time_block, s, C.byref(res)))
+    torch.cuda.synchronize()
+    got0 = (b if res.value else a).cpu().numpy().view(np.uint16)
+    assert np.array_equal(got0, ref0)
