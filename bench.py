@@ -18,22 +18,28 @@ idle host work between legs, which would cost the next 100-step run
 12-15 %.
 
 One JSON line: value = FP64 cell-updates/s at the best time-blocking depth
-k (steps fused per launch; bit-exact with the reference for every k,
-checked against its 512^3 x 100 checksum), plus variants:
-  {f64,f32,bf16}_k{k}  the workload per storage type and depth
+k (steps fused per launch), plus variants:
+  {f64,f32,bf16}_k{k}  the workload per storage type and depth; every k's
+                   final field is compared bit for bit with k = 1's
   mixed_gauss1024  1024^3 x 200 Gaussian bubble in FP64/FP32/BF16 (round to
                    nearest and stochastic): rates, max-abs / RMS / max-rel
                    error vs FP64, mass drift
   fwd_adj_*        forward + adjoint (dL/df0, dL/dv of a misfit) at 512^3
-                   over the same steps, time ratio to the one-step forward
+                   over the same steps, by the shift-invariant and the
+                   checkpointed method; time ratio to the best forward
   adjoint_step     one adjoint step per storage type (24 / 12 / 6 B a cell)
-  adjoint_large    1024x1024x768 x 400 checkpointed gradient, BF16-SR/FP32
-                   against FP64
+  adjoint_large    1024x1024x768 x 400 gradient, BF16-SR/FP32 against FP64
   laplacian        the stencil filter (cxx/Filter.cpp) on laplacian.cxx's
                    2-D 8000^2 and a 3-D 512^3 Laplacian, 10 iterations
-and the roofline of the dominant kernel and the CPU baseline (the
-reference's own upwind.cxx, compiled by oracle/Makefile, timed on this
-host's cores on a bounded sample of the same workload).
+  copy             the device's plain-copy bandwidth (torch's elementwise
+                   kernel over 1 GiB), the ceiling of one-pass kernels and
+                   the calibration of the PMC byte counters
+and parity (never null: the GPU's FP64 checksum against the reference's at
+the CPU leg's step count, against the reference's 512^3 x 100 checksum,
+and every k against k = 1), the roofline of the dominant kernel, and the
+CPU baseline (the reference's own upwind.cxx, compiled by oracle/Makefile,
+timed on this host's cores on a bounded sample of the same workload,
+median of 3).
 """
 from __future__ import annotations
 
@@ -51,7 +57,11 @@ sys.path.insert(0, ROOT)
 METRIC = "cell-updates/s & %HBM roofline at 512³: FP64 plain vs FP32/BF16 vs fwd+adjoint"
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 BYTES = {"f64": 8, "f32": 4, "bf16": 2, "bf16sr": 2}
-ALL = ("f64", "f32", "bf16", "mixed", "fwdadj", "adjlarge", "laplacian")
+ALL = ("f64", "f32", "bf16", "mixed", "fwdadj", "adjlarge", "laplacian", "copy")
+# VALU lane-operations per second the clock allows: 256 CUs x 4 SIMDs x 16
+# lanes x 2.4 GHz (FP64 add/mul and non-packed FP32 issue one lane-op per
+# lane per cycle; a probe of independent adds/multiplies reached 35.5 T).
+VALU_PEAK = 256 * 4 * 16 * 2.4e9
 
 
 def parse():
@@ -77,42 +87,49 @@ def sync():
 
 # ---------------------------------------------------------------- CPU leg
 
-def _cpu_advect_run(w, threads, budget_s):
+def _cpu_advect_run(w, threads, budget_s, steps=None):
     """One timed advect of the reference (oracle/_ref) -- or the oracle port
     when the reference harness was not built -- in a child process with
     `threads` OpenMP threads, on a sample of the workload (same field,
-    velocity and dt) of as many steps as fit in about budget_s."""
+    velocity and dt): `steps` steps, or as many as fit in about budget_s.
+    Returns kind, steps, seconds, threads and the checksum of the result."""
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
                OMP_PLACES="cores")
     code = r"""
 import json, sys, time
 sys.path.insert(0, %(root)r)
 from oracle import oracle as O
-n, v, dt, p, seed, budget = %(args)r
+n, v, dt, p, seed, budget, steps = %(args)r
 f0 = O.sphere_noise(n, (1.0, 1.0, 1.0), p[:3], p[3], p[4], seed)
 kind = "reference" if O.have_ref() else "port"
-if kind == "reference":
-    t1, _ = O.ref_time_advect(f0, v, (1.0, 1.0, 1.0), dt, 1)
+def run(k):
+    if kind == "reference":
+        return O.ref_time_advect(f0, v, (1.0, 1.0, 1.0), dt, k)
+    t0 = time.perf_counter(); f = O.advect(f0, v, (1.0, 1.0, 1.0), dt, k)
+    return time.perf_counter() - t0, O.checksum(f)
+if steps is None:
+    t1, _ = run(1)
     steps = max(1, min(50, int(budget / max(t1, 1e-3))))
-    t, cs = O.ref_time_advect(f0, v, (1.0, 1.0, 1.0), dt, steps)
-else:
-    t0 = time.perf_counter(); O.advect(f0, v, (1.0, 1.0, 1.0), dt, 1); t1 = time.perf_counter() - t0
-    steps = max(1, min(50, int(budget / max(t1, 1e-3))))
-    t0 = time.perf_counter(); O.advect(f0, v, (1.0, 1.0, 1.0), dt, steps); t = time.perf_counter() - t0
-print(json.dumps(dict(kind=kind, steps=steps, seconds=t, threads=O.lib().orc_num_threads())))
+t, cs = run(steps)
+print(json.dumps(dict(kind=kind, steps=steps, seconds=t, checksum=repr(cs),
+                      threads=O.lib().orc_num_threads())))
 """ % dict(root=ROOT, args=(list(w.num_cells), list(w.velocity), w.dt, list(w.init_params),
-                            w.init_seed, budget_s))
+                            w.init_seed, budget_s, steps))
     out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True,
                          text=True, timeout=600, check=True).stdout.strip().splitlines()[-1]
     return json.loads(out)
 
 
-def cpu_baseline(w, budget_s=12.0, single_budget_s=6.0):
-    """The CPU figure on this host: the reference's advect on all its cores
-    (about budget_s of work), plus the same on one core (single_core)."""
+def cpu_baseline(w, budget_s=4.0, reps=3, single_budget_s=6.0):
+    """The CPU figure on this host: the reference's advect on all its cores,
+    the median of `reps` runs of the same sample (about budget_s each), plus
+    one run on one core (single_core).  Also returns the sample's step count
+    and reference checksum, which the parity check reproduces on the GPU."""
     ncores = len(os.sched_getaffinity(0))
     try:
-        r = _cpu_advect_run(w, ncores, budget_s)
+        runs = [_cpu_advect_run(w, ncores, budget_s)]
+        for _ in range(reps - 1):
+            runs.append(_cpu_advect_run(w, ncores, budget_s, steps=runs[0]["steps"]))
     except Exception as e:  # the baseline is reported, never required
         return {"value": None, "unit": "cell-updates/s", "cores": ncores, "kind": None,
                 "sample": f"failed: {e}"}
@@ -122,11 +139,17 @@ def cpu_baseline(w, budget_s=12.0, single_budget_s=6.0):
             model = next(l.split(":", 1)[1].strip() for l in fh if l.startswith("model name"))
     except Exception:
         pass
-    out = {"value": w.ntot * r["steps"] / r["seconds"], "unit": "cell-updates/s",
-           "cores": r["threads"], "kind": r["kind"],
-           "sample": (f"{w.num_cells[0]}^3 x {r['steps']} steps of the same workload "
-                      f"({r['seconds']:.2f} s, advect only, OMP_NUM_THREADS={r['threads']}, "
-                      f"{model})")}
+    rates = sorted(w.ntot * r["steps"] / r["seconds"] for r in runs)
+    r0 = runs[0]
+    secs = ", ".join("%.2f" % r["seconds"] for r in runs)
+    out = {"value": rates[len(rates) // 2], "unit": "cell-updates/s",
+           "cores": r0["threads"], "kind": r0["kind"], "runs": rates,
+           "sample": (f"{w.num_cells[0]}^3 x {r0['steps']} steps of the same workload, median "
+                      f"of {len(runs)} runs ({secs} s, advect only, "
+                      f"OMP_NUM_THREADS={r0['threads']}, {model})"),
+           "steps": r0["steps"], "checksum": r0["checksum"]}
+    if any(r["checksum"] != r0["checksum"] for r in runs):
+        out["checksum_mismatch_between_runs"] = [r["checksum"] for r in runs]
     try:
         r1 = _cpu_advect_run(w, 1, single_budget_s)
         out["single_core"] = {"value": w.ntot * r1["steps"] / r1["seconds"],
@@ -187,6 +210,35 @@ def run_forward(w, dtype, steps, warmup, time_block):
     return ms, launches, up
 
 
+class _DevArray:
+    """A device buffer seen by torch through __cuda_array_interface__."""
+
+    def __init__(self, ptr, n, typestr):
+        self.__cuda_array_interface__ = {"shape": (n,), "typestr": typestr,
+                                         "data": (ptr, False), "version": 3}
+
+
+def field_bits(up):
+    """The handle's current field as a torch integer tensor of its bits (no copy)."""
+    import torch
+    ts = {8: "<i8", 4: "<i4", 2: "<i2"}[BYTES[{0: "f64", 1: "f32", 2: "bf16", 3: "bf16"}[up.dtype]]]
+    return torch.as_tensor(_DevArray(up.device_ptr(), up.ntot, ts), device="cuda")
+
+
+def same_bits(a_up, b_up):
+    import torch
+    return bool(torch.equal(field_bits(a_up), field_bits(b_up)))
+
+
+def kernel_launch_ms(w, dtype, k, launches=12):
+    """Average duration of one launch of the k-step kernel (k = 1: the
+    one-step kernel): `launches` launches of exactly k steps each, HIP events
+    on the handle's stream around them, after the clock warm-up."""
+    ms, n, up = run_forward(w, dtype, k * launches, 0, k)
+    up.close()
+    return ms / n, n
+
+
 def compare(a_up, b_up):
     """{max_abs, rms, max_rel} of handle a's field against handle b's."""
     from fidibench_amd import _lib as L
@@ -228,13 +280,13 @@ def mixed_precision(n=1024, steps=200):
     return res
 
 
-def forward_adjoint(w, steps, f_dtype, fwd_ms):
-    """Forward (keeping checkpoints) + backward of a misfit loss over
-    `steps` steps at the workload's size; time ratio to the forward alone."""
+def forward_adjoint(w, steps, f_dtype, fwd_ms, method):
+    """Forward + backward of a misfit loss over `steps` steps at the
+    workload's size through the tape (method "shift" or "checkpoint"); time
+    ratio to the forward alone (fwd_ms: {name: ms of a plain forward})."""
     import torch
     from fidibench_amd import _lib as L
     from fidibench_amd.autograd import Tape, default_budget
-    from fidibench_amd import configs as K
     tdt = {"f64": torch.float64, "f32": torch.float32, "bf16": torch.bfloat16}[f_dtype]
     fd = {"f64": L.F64, "f32": L.F32, "bf16": L.BF16}[f_dtype]
     f0 = torch.empty(w.num_cells, dtype=tdt, device="cuda")
@@ -243,7 +295,8 @@ def forward_adjoint(w, steps, f_dtype, fwd_ms):
                                        f0.data_ptr(), torch.cuda.current_stream().cuda_stream))
     fT = torch.empty_like(f0)
     lam0 = torch.empty_like(f0)
-    tape = Tape(w.num_cells, w.velocity, w.lengths, w.dt, steps, fd, fd, default_budget(0.8))
+    tape = Tape(w.num_cells, w.velocity, w.lengths, w.dt, steps, fd, fd, default_budget(0.8),
+                method=method)
     e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
     sync()
     gpu_busy()
@@ -258,11 +311,14 @@ def forward_adjoint(w, steps, f_dtype, fwd_ms):
     tot = e0.elapsed_time(e2)
     st = tape.stats()
     tape.close()
-    return {"ms_forward_with_checkpoints": fwd, "ms_forward_plus_adjoint": tot,
-            "ratio_to_plain_forward": tot / fwd_ms if fwd_ms else None,
-            "cell_steps_per_s": w.ntot * steps / (tot * 1e-3), "snapshots": st["snapshots"],
-            "forward_steps": st["forward_steps"], "adjoint_steps": st["adjoint_steps"],
-            "grad_v": gv}
+    out = {"method": method, "ms_forward": fwd, "ms_forward_plus_adjoint": tot,
+           "cell_steps_per_s": w.ntot * steps / (tot * 1e-3), "snapshots": st["snapshots"],
+           "forward_steps": st["forward_steps"], "adjoint_steps": st["adjoint_steps"],
+           "grad_v": gv}
+    for name, ms in fwd_ms.items():
+        if ms:
+            out[f"ratio_to_{name}"] = tot / ms
+    return out
 
 
 def adjoint_step_rates(w, reps=10):
@@ -366,9 +422,11 @@ def laplacian(iters=10, with_cpu=True):
 
 
 def large_adjoint(shape=(1024, 1024, 768), steps=400):
-    """The checkpointed gradient at 1024x1024x768 x 400 steps: BF16-storage
-    forward with FP32 adjoint, against the same gradient in FP64; time
-    ratio to the plain BF16 forward over the same steps."""
+    """The gradient at 1024x1024x768 x 400 steps: BF16-SR forward with FP32
+    adjoint by both methods (checkpointed: snapshots sized to the free HBM),
+    against the same gradient in FP64; time ratios to the plain BF16
+    forwards over the same steps (round to nearest fused 8 steps per pass,
+    and the stochastic-rounding forward the gradient runs)."""
     import torch
     from fidibench_amd import _lib as L, configs as K
     from fidibench_amd.workloads import advect, compare, init_field, misfit_gradient
@@ -376,31 +434,76 @@ def large_adjoint(shape=(1024, 1024, 768), steps=400):
     wt = K.Workload("misfit_large_target", w.num_cells, steps, (0.8, 0.3, -0.5), w.init_kind,
                     w.init_params, dt_velocity=w.dt_velocity)
     target = advect(init_field(wt, L.F64), wt.velocity, wt, steps)
-    free = torch.cuda.mem_get_info()[0]
-    g64 = misfit_gradient(w, target, L.F64, L.F64, int(free * 0.7))
+    g64 = misfit_gradient(w, target, L.F64, L.F64, 64 << 30, method="shift")
     lam64, gv64 = g64.pop("grad_f0"), g64.pop("grad_v")
-    free = torch.cuda.mem_get_info()[0]
-    g16 = misfit_gradient(w, target, L.BF16_SR, L.F32, int(free * 0.8))
-    lam16, gv16 = g16.pop("grad_f0"), g16.pop("grad_v")
-    err = compare(lam16, lam64)
-    del lam16, lam64
-    f0 = init_field(w, L.BF16)
+    fwd = {}
+    for name, code, tb in (("plain_bf16_forward_s", L.BF16, 8),
+                           ("plain_bf16sr_forward_s", L.BF16_SR, 1)):
+        f0 = init_field(w, L.BF16)
+        sync()
+        gpu_busy()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        _advect_code(f0, w, steps, tb, code)
+        e1.record()
+        sync()
+        fwd[name] = e0.elapsed_time(e1) * 1e-3
+        del f0
+    out = {"workload": (f"misfit_large: {shape[0]}x{shape[1]}x{shape[2]}, {steps} steps, "
+                        "8 Gaussian bubbles a*exp(-|x-c|^2/w^2), v=(0.9,0.25,-0.45), target "
+                        "at v*=(0.8,0.3,-0.5)"),
+           "fp64_shift": g64, "grad_v_fp64": gv64, **fwd}
+    for method in ("shift", "checkpoint"):
+        free = torch.cuda.mem_get_info()[0]
+        sync()
+        gpu_busy()
+        g = misfit_gradient(w, target, L.BF16_SR, L.F32, int(free * 0.8), method=method)
+        lam, gv = g.pop("grad_f0"), g.pop("grad_v")
+        g["grad_f0_error_vs_fp64"] = compare(lam, lam64)
+        g["grad_v_rel_error_vs_fp64"] = [abs(a - b) / abs(b) for a, b in zip(gv, gv64)]
+        for name, sec in fwd.items():
+            g["ratio_to_" + name[:-2]] = g["seconds"] / sec
+        out[f"bf16sr_fp32_{method}"] = g
+        del lam
+    return out
+
+
+def _advect_code(f0, w, nsteps, time_block, code):
+    """nsteps forward steps of a torch field in place of storage type `code`
+    (FDB_BF16_SR keyed from step 0, seed 0)."""
+    import torch
+    from fidibench_amd import _lib as L
+    a, b = f0, torch.empty_like(f0)
+    res = C.c_int()
+    L.check(L.lib().fdb_advect_dev(3, L.u64s(w.num_cells), L.dbls(w.velocity), L.dbls(w.lengths),
+                                   w.dt, code, a.data_ptr(), b.data_ptr(), int(nsteps),
+                                   int(time_block), torch.cuda.current_stream().cuda_stream,
+                                   C.byref(res)))
+    return b if res.value else a
+
+
+def copy_bandwidth(nbytes=1 << 30, reps=10):
+    """The device's plain-copy rate: torch's elementwise kernel b = a * 1.0
+    over `nbytes` of FP64 (reads and writes nbytes once per launch), HIP
+    events around `reps` launches on torch's stream.  A rocprofv3 PMC run of
+    this leg also calibrates FETCH_SIZE / WRITE_SIZE against known bytes."""
+    import torch
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.mul(a, 1.0, out=b)
     sync()
     gpu_busy()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
-    advect(f0, w.velocity, w, steps, 1)
+    for _ in range(reps):
+        torch.mul(a, 1.0, out=b)
     e1.record()
     sync()
-    fwd_s = e0.elapsed_time(e1) * 1e-3
-    return {"workload": (f"misfit_large: {shape[0]}x{shape[1]}x{shape[2]}, {steps} steps, "
-                         "8 Gaussian bubbles a*exp(-|x-c|^2/w^2), v=(0.9,0.25,-0.45), target "
-                         "at v*=(0.8,0.3,-0.5)"),
-            "bf16_fp32": {**g16, "ratio_to_plain_bf16_forward": g16["seconds"] / fwd_s},
-            "fp64": g64, "plain_bf16_forward_s": fwd_s,
-            "grad_f0_error_vs_fp64": err,
-            "grad_v_rel_error_vs_fp64": [abs(a - b) / abs(b) for a, b in zip(gv16, gv64)],
-            "grad_v_fp64": gv64}
+    ms = e0.elapsed_time(e1) / reps
+    return {"bytes_read_per_launch": nbytes, "bytes_written_per_launch": nbytes,
+            "avg_launch_ms": ms, "GB_per_s": 2 * nbytes / (ms * 1e-3) / 1e9,
+            "frac_of_peak": 2 * nbytes / (ms * 1e-3) / HBM_PEAK,
+            "kernel": "torch elementwise mul (b = a * 1.0, FP64)"}
 
 
 def pmc_traffic(kernel_key, summary="pmc_traffic.txt"):
@@ -414,6 +517,55 @@ def pmc_traffic(kernel_key, summary="pmc_traffic.txt"):
     except OSError:
         pass
     return None
+
+
+def fp64_checksum(w, steps, k):
+    """The reference's checksum (host left fold) of the FP64 GPU field after
+    `steps` steps of the workload at time-blocking depth k (untimed)."""
+    from fidibench_amd import Upwind
+    up = Upwind(w.velocity, w.lengths, w.num_cells)
+    up.set_time_block(k)
+    up.init_field(w.init_kind, w.init_params, seed=w.init_seed)
+    up.advect(steps, w.dt)
+    cs = up.checksum()
+    up.close()
+    return cs
+
+
+def roofline(w, dtype, k, steps):
+    """Roofline of the dominant kernel: the k-step forward kernel of
+    `dtype` (k = 1: the one-step kernel).  Its average launch duration is
+    measured here (kernel_launch_ms).  frac is a position on the HBM
+    roofline: compulsory bytes per launch (read state t, write t+k: 2*s per
+    cell) / duration / 8 TB/s.  effective_frac credits 2*s bytes to every
+    fused step (k x compulsory), the bytes k one-step launches would move.
+    traffic: HBM bytes per launch from the committed rocprofv3 PMC summary
+    (FETCH_SIZE, WRITE_SIZE, calibrated on the copy leg).  valu: vector-ALU
+    lane-operations issued per second (SQ_INSTS_VALU x 64 per launch, from
+    the committed summary) against the 39.3 T the clock allows."""
+    s = BYTES[dtype]
+    launch_ms, _ = kernel_launch_ms(w, dtype, k)
+    comp = 2.0 * s * w.ntot
+    achieved = comp / (launch_ms * 1e-3)
+    out = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK, "traffic": pmc_traffic(f"{dtype}_k{k}"),
+           "kernel": (f"tb_kernel<{ {'f64': 'double', 'f32': 'float'}.get(dtype, 'bf16')}> "
+                      f"({dtype.upper()} forward, {k} fused steps per launch)" if k > 1 else
+                      f"rows_kernel ({dtype.upper()} forward, 1 step per launch)"),
+           "steps_per_launch": k, "avg_launch_ms": launch_ms,
+           "algorithmic_bytes_per_launch": comp,
+           "note": ("algorithmic bytes = compulsory bytes per launch: read state t and write "
+                    "state t+k once (2*s per cell); effective_* credit them to each of the k "
+                    "fused steps")}
+    out["effective_achieved"] = k * achieved / 1e9
+    out["effective_frac"] = k * achieved / HBM_PEAK
+    valu = pmc_traffic(f"valu_{dtype}_k{k}")
+    if valu:
+        rate = valu * 64 / (launch_ms * 1e-3)
+        out["valu"] = {"lane_ops_per_s": rate, "peak": VALU_PEAK, "frac": rate / VALU_PEAK,
+                       "instructions_per_launch": valu,
+                       "instructions_per_cell_step": valu * 64 / (w.ntot * k)}
+    return out
 
 
 def main():
@@ -430,28 +582,36 @@ def main():
     global WARM_MS
     WARM_MS = a.device_warmup_ms
     res = {}
-    best = None
+    parity = {"checks": {}}
 
     # Every storage type at every time-blocking depth k (k fused steps per
-    # launch; k = 1 is one launch per step); results are identical for all
-    # k, and the FP64 runs are checked against the reference's checksum.
+    # launch; k = 1 is one launch per step).  Every k's final field must
+    # equal k = 1's bit for bit (compared on the device).
     ks = [int(x) for x in a.time_blocks.split(",")]
+    if 1 not in ks:
+        ks = [1] + ks
     best_of = {}
     for dt_name in ("f64", "f32", "bf16"):
         if dt_name not in only:
             continue
+        ref_up = None
+        same = {}
         for k in ks:
             ms, launches, up = run_forward(w, dt_name, a.steps, a.warmup, k)
             cu = w.ntot * a.steps / (ms * 1e-3)
-            entry = {"cell_updates_per_s": cu, "ms": ms, "launches": launches,
-                     "time_block": k,
-                     "hbm_frac_compulsory": cu * 2 * BYTES[dt_name] / HBM_PEAK}
-            if dt_name == "f64" and a.steps == 100 and a.n == 512:
-                entry["checksum"] = up.checksum()
-            res[f"{dt_name}_k{k}"] = entry
+            res[f"{dt_name}_k{k}"] = {"cell_updates_per_s": cu, "ms": ms, "launches": launches,
+                                      "time_block": k,
+                                      "hbm_frac_compulsory": cu * 2 * BYTES[dt_name] / HBM_PEAK}
             if dt_name not in best_of or cu > best_of[dt_name][0]:
                 best_of[dt_name] = (cu, k, ms, launches)
-            up.close()
+            if ref_up is None:
+                ref_up = up
+            else:
+                same[k] = same_bits(up, ref_up)
+                res[f"{dt_name}_k{k}"]["bit_identical_to_k1"] = same[k]
+                up.close()
+        ref_up.close()
+        parity["checks"][f"{dt_name}_every_k_equals_k1"] = all(same.values())
     best = best_of.get("f64")
     for dt_name in ("f32", "bf16"):
         if dt_name in best_of and best:
@@ -459,30 +619,19 @@ def main():
     if "mixed" in only:
         res["mixed_gauss1024"] = mixed_precision()
     if "fwdadj" in only:
-        # The ratio is to the plain one-step forward over the same steps.
-        fwd_ms = res.get("f64_k1", {}).get("ms")
-        res["fwd_adj_f64"] = forward_adjoint(w, a.steps, "f64", fwd_ms)
-        f32_ms = res.get("f32_k1", {}).get("ms")
-        res["fwd_adj_f32"] = forward_adjoint(w, a.steps, "f32", f32_ms)
-        bf16_ms = res.get("bf16_k1", {}).get("ms")
-        res["fwd_adj_bf16"] = forward_adjoint(w, a.steps, "bf16", bf16_ms)
+        for dt_name in ("f64", "f32", "bf16"):
+            fwd = {"best_forward": best_of.get(dt_name, (0, 0, None))[2],
+                   "one_step_forward": res.get(f"{dt_name}_k1", {}).get("ms")}
+            for method in ("shift", "checkpoint"):
+                key = f"fwd_adj_{dt_name}" + ("" if method == "shift" else "_checkpoint")
+                res[key] = forward_adjoint(w, a.steps, dt_name, fwd, method)
         res["adjoint_step"] = adjoint_step_rates(w)
     if "adjlarge" in only:
         res["adjoint_large"] = large_adjoint()
     if "laplacian" in only:
         res["laplacian"] = laplacian(with_cpu=not a.no_cpu)
-
-    parity = None
-    try:
-        with open(os.path.join(ROOT, "tests", "golden", "manifest.json")) as fh:
-            gold = json.load(fh)["large"].get("sphere_noise512")
-        got = [v.get("checksum") for k, v in res.items() if k.startswith("f64_k")]
-        if gold and got and all(g is not None for g in got):
-            parity = {"reference_checksum": gold["checksum"],
-                      "checksums": [repr(g) for g in got],
-                      "bit_exact": all(repr(g) == gold["checksum"] for g in got)}
-    except Exception:
-        pass
+    if "copy" in only:
+        res["copy"] = copy_bandwidth()
 
     out = {"metric": METRIC, "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps,
            "warmup": a.warmup, "higher_is_better": True, "scaling": "weak",
@@ -492,31 +641,39 @@ def main():
         cu, k, ms, launches = best
         out["value"] = cu
         out["ms_per_step"] = ms / a.steps
-        per_launch_ms = ms / launches
-        steps_per_launch = a.steps / launches
-        alg_bytes = 16.0 * w.ntot * steps_per_launch
-        achieved = alg_bytes / (per_launch_ms * 1e-3)
-        out["roofline"] = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                           "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                           "traffic": pmc_traffic(f"f64_k{k}"),
-                           "kernel": (f"tb_kernel<double> (FP64 forward, {k} fused steps per "
-                                      "launch)" if k > 1 else
-                                      "rows_kernel<double> (FP64 forward, 1 step per launch)"),
-                           "steps_per_launch": steps_per_launch,
-                           "note": ("algorithmic bytes count 16 B per cell-update (read t, "
-                                    "write t+1) for every fused step, so frac > 1 is possible "
-                                    "with k > 1; traffic is the measured HBM bytes per launch"),
-                           "algorithmic_bytes_per_launch": alg_bytes,
-                           "avg_launch_ms": per_launch_ms}
+        out["time_block"] = k
+        out["roofline"] = roofline(w, "f64", k, a.steps)
     out["config"] = {"workload": f"sphere_noise{a.n}: {a.n}^3 cells, {a.steps} steps, FP64, "
                                  "CFL 0.5, seeded mixed-sign velocity, sphere+0.01 noise",
                      "num_cells": list(w.num_cells), "velocity": list(w.velocity), "dt": w.dt}
     out["variants"] = res
-    out["parity"] = parity
+    k_best = best[1] if best else 8
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(w)
-        if out["cpu_baseline"].get("value") and out.get("value"):
-            out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+        cpu = cpu_baseline(w)
+        out["cpu_baseline"] = cpu
+        if cpu.get("value") and out.get("value"):
+            out["gpu_over_cpu"] = out["value"] / cpu["value"]
+        if cpu.get("checksum"):
+            # The GPU from the same field for the CPU sample's step count,
+            # against the checksum the reference computed in that leg.
+            g = fp64_checksum(w, cpu["steps"], k_best)
+            parity["cpu_leg"] = {"steps": cpu["steps"], "reference_checksum": cpu["checksum"],
+                                 "gpu_checksum": repr(g), "kind": cpu["kind"]}
+            parity["checks"]["cpu_leg_checksum"] = repr(g) == cpu["checksum"]
+    try:
+        with open(os.path.join(ROOT, "tests", "golden", "manifest.json")) as fh:
+            gold = json.load(fh)["large"].get(f"sphere_noise{a.n}")
+    except (OSError, ValueError, KeyError):
+        gold = None
+    if gold:
+        # The reference's checksum of this workload at 100 steps (fixture
+        # written by the reference itself), reproduced at the best k.
+        g = fp64_checksum(w, 100, k_best)
+        parity["golden_100_steps"] = {"reference_checksum": gold["checksum"],
+                                      "gpu_checksum": repr(g), "time_block": k_best}
+        parity["checks"]["golden_100_steps"] = repr(g) == gold["checksum"]
+    parity["bit_exact"] = bool(parity["checks"]) and all(parity["checks"].values())
+    out["parity"] = parity
     print(json.dumps(out), flush=True)
 
 

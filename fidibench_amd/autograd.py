@@ -46,6 +46,14 @@ class Tape:
         self.ndims = len(self.shape)
         self.f_dtype, self.lam_dtype = f_dtype, lam_dtype
         h = C.c_void_p()
+        if method == "shift":
+            # No snapshots in this method: ask for the fixed working set only
+            # (initial state, two working buffers, one adjoint buffer).
+            ntot = 1
+            for n in self.shape:
+                ntot *= n
+            fixed = ntot * (3 * L.ITEMSIZE[f_dtype] + L.ITEMSIZE[lam_dtype])
+            max_bytes = min(int(max_bytes), fixed)
         L.check(L.lib().fdb_tape_create(self.ndims, L.u64s(self.shape), L.dbls(velocity),
                                         L.dbls(lengths), float(dt), int(nsteps), f_dtype,
                                         lam_dtype, int(max_bytes), C.byref(h)))

@@ -139,7 +139,7 @@ struct fdb_tape {
   bool fuse_adj = false;  // this backward may use fused transpose passes
   int64_t fwd_steps = 0, adj_steps = 0;
   int method = FDB_GRAD_CHECKPOINT;
-  bool slots_ready = false;    // snapshot slots are allocated at the first checkpointed forward
+  bool slots_ready = false;    // snapshot slots allocated (released by FDB_GRAD_SHIFT)
   const void* f0_ref = nullptr;  // FDB_GRAD_SHIFT: the caller's f0
 };
 
@@ -502,6 +502,9 @@ int fdb_tape_create(int ndims, const uint64_t* num_cells, const double* velocity
     if ((e = hipMalloc(&w, t->fbytes)) != hipSuccess) return fail_free(e, "hipMalloc scratch");
   if ((e = hipMalloc(&t->lam_tmp, t->lbytes)) != hipSuccess) return fail_free(e, "hipMalloc lam");
   t->slots.assign(t->snaps, nullptr);
+  for (void*& sl : t->slots)
+    if ((e = hipMalloc(&sl, t->fbytes)) != hipSuccess) return fail_free(e, "hipMalloc snapshot");
+  t->slots_ready = true;
   if ((e = hipMalloc((void**)&t->acc, 6 * sizeof(double))) != hipSuccess)
     return fail_free(e, "hipMalloc acc");
   *out = t;
@@ -524,7 +527,7 @@ int fdb_tape_forward(fdb_tape_t* t, const void* f0, void* fT, void* stream) {
     t->forward_done = true;
     return FDB_OK;
   }
-  if (!t->slots_ready) {
+  if (!t->slots_ready) {  // released by an earlier switch to FDB_GRAD_SHIFT
     for (void*& sl : t->slots) TAPE_HIP(hipMalloc(&sl, t->fbytes));
     t->slots_ready = true;
   }
@@ -605,6 +608,16 @@ int fdb_tape_set_method(fdb_tape_t* t, int method) {
     return tfail(FDB_ERR_INVALID, "tape_set_method: unknown method");
   if (method != t->method) t->forward_done = false;  // a backward needs this method's forward
   t->method = method;
+  if (method == FDB_GRAD_SHIFT && t->slots_ready) {
+    // No snapshots in this method: give their memory back (a later
+    // checkpointed forward allocates them again).
+    TAPE_HIP(hipDeviceSynchronize());
+    for (void*& sl : t->slots) {
+      if (sl) TAPE_HIP(hipFree(sl));
+      sl = nullptr;
+    }
+    t->slots_ready = false;
+  }
   return FDB_OK;
 }
 

@@ -80,7 +80,7 @@ def test_large_gradient_bf16_forward_fp32_adjoint():
     wt = K.Workload("misfit_large_target", w.num_cells, w.steps, (0.8, 0.3, -0.5), w.init_kind,
                     w.init_params, dt_velocity=w.dt_velocity)
     target = advect(init_field(wt, L.F64), wt.velocity, wt, wt.steps)
-    g64 = misfit_gradient(w, target, L.F64, L.F64, 16 << 30, method="shift")
+    g64 = misfit_gradient(w, target, L.F64, L.F64, 64 << 30, method="shift")
     lam64, gv64 = g64.pop("grad_f0"), np.array(g64.pop("grad_v"))
     fbytes = w.ntot * 2
     free = torch.cuda.mem_get_info()[0]
