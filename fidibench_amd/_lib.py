@@ -58,6 +58,8 @@ SIGNATURES = {
                                    C.c_uint64, C.c_uint64, _vp, C.POINTER(_i)]),
     "fdb_adjoint_step_dev": (_i, [_i, _u64p, _dp, _dp, C.c_double, _i, _vp, _vp, _i, _vp, _vp,
                                   _vp]),
+    "fdb_adjoint_step_dev_noalloc": (_i, [_i, _u64p, _dp, _dp, C.c_double, _i, _vp, _vp, _i,
+                                          _vp, _vp, _vp]),
     "fdb_sum_dev": (_i, [_i, _vp, C.c_uint64, _dp, _vp]),
     "fdb_misfit_dev": (_i, [_i, _vp, _vp, C.c_uint64, _i, _vp, _dp, _vp]),
     "fdb_tape_create": (_i, [_i, _u64p, _dp, _dp, C.c_double, _i, _i, _i, C.c_uint64,

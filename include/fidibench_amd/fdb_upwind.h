@@ -170,6 +170,21 @@ int fdb_adjoint_step_dev(int ndims, const uint64_t* num_cells, const double* vel
                          const void* lam_next, void* lam, int f_dtype, const void* f_t,
                          double* vgrad_acc, void* stream);
 
+/* fdb_adjoint_step_dev without any device allocation, for callers that
+ * cannot allocate: code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc.  No counterpart in the reference.  Same arguments
+ * and results (lam and the vgrad_acc sums bit-identical); the velocity
+ * products' per-block partial sums live in a fixed device array of 65536
+ * slots (3 MiB, held by each GPU the library runs on until the process
+ * ends) instead of a hipMalloc'd buffer, so a grid needing more slots is
+ * FDB_ERR_INVALID (512^3 needs at most a few thousand).  Returns once the
+ * work is enqueued.  All calls on one GPU share the array: calls ordered on
+ * one stream are safe, calls that overlap on one GPU race. */
+int fdb_adjoint_step_dev_noalloc(int ndims, const uint64_t* num_cells, const double* velocity,
+                                 const double* lengths, double delta_time, int lam_dtype,
+                                 const void* lam_next, void* lam, int f_dtype, const void* f_t,
+                                 double* vgrad_acc, void* stream);
+
 /* FP64 compensated sum of n values (dtype) at x; blocks until done. */
 int fdb_sum_dev(int dtype, const void* x, uint64_t n, double* out, void* stream);
 
