@@ -80,12 +80,28 @@ __device__ inline uint32_t sr_pair_hash(uint64_t pair, uint32_t key) {
   return lowbias32((uint32_t)(pair ^ (pair >> 32)) ^ key);
 }
 
-// x rounded with 16 random bits r (r < 2^16).
-__device__ inline uint16_t bf16_sr_bits(float x, uint32_t r) {
-  const uint32_t b = __builtin_bit_cast(uint32_t, x);
-  if ((b & 0x7f800000u) == 0x7f800000u)
-    return (uint16_t)((b >> 16) | ((b & 0x007fffffu) ? 0x0040u : 0u));
-  return (uint16_t)((b + r) >> 16);
+// x rounded with 16 random bits r16: (bits(x) + r16) >> 16 for finite x
+// (a carry into the exponent rounds up to the next binade, or to Inf past
+// the largest finite value), Inf kept, NaN kept quiet.  gfx950's
+// v_cvt_sr_bf16_f32 does exactly this with the random bits taken from bits
+// 31:16 of its second operand (checked against this rule over 10^6 random
+// values and every special class: NaN payloads, +-Inf, +-0, denormals,
+// the largest finite value), so it is called with r16 << 16.
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+__device__ inline uint16_t bf16_sr_bits(float x, uint32_t r16) {
+  const s16x2 z = {0, 0};
+  return (uint16_t)__builtin_bit_cast(uint32_t,
+                                      __builtin_amdgcn_cvt_sr_bf16_f32(z, x, r16 << 16, false));
+}
+
+// Two values of one pair rounded with the pair's hash h: the even cell
+// with its low 16 bits, the odd cell with its high 16 bits; packed low =
+// even.  Two conversions and a shift.
+__device__ inline uint32_t bf16x2_sr(float even, float odd, uint32_t h) {
+  s16x2 v = {0, 0};
+  v = __builtin_amdgcn_cvt_sr_bf16_f32(v, even, h << 16, false);
+  v = __builtin_amdgcn_cvt_sr_bf16_f32(v, odd, h, true);
+  return __builtin_bit_cast(uint32_t, v);
 }
 
 __device__ inline uint16_t bf16_sr(float x, uint64_t cell, uint32_t key) {
@@ -485,12 +501,11 @@ __device__ inline void store_pack_sr(uint16_t* p, const float (&in)[VEC], uint64
     // cell is even here (n2 % VEC == 0 and the pack starts a VEC group):
     // one hash serves the pair (cell + v, cell + v + 1).
     const uint64_t pair0 = cell >> 1;
+    uint32_t w[VEC / 2];
 #pragma unroll
-    for (int v = 0; v < VEC; v += 2) {
-      const uint32_t h = sr_pair_hash(pair0 + (uint64_t)(v / 2), key);
-      pk.v[v] = bf16_sr_bits(in[v], h & 0xffffu);
-      pk.v[v + 1] = bf16_sr_bits(in[v + 1], h >> 16);
-    }
+    for (int v = 0; v < VEC; v += 2)
+      w[v / 2] = bf16x2_sr(in[v], in[v + 1], sr_pair_hash(pair0 + (uint64_t)(v / 2), key));
+    pk = __builtin_bit_cast(Pack<uint16_t, VEC>, w);
   } else {
 #pragma unroll
     for (int v = 0; v < VEC; ++v) pk.v[v] = bf16_sr(in[v], cell + v, key);
@@ -837,6 +852,13 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
                                                      Coef<typename St<TS>::M> cf, TbAux adj) {
   using M = typename St<TS>::M;
   constexpr bool PROD = MODE == kTbAdjoint, ALL = MODE == kTbStoreAll;
+  // FP64 forward passes exchange the edge rows one level ahead (PIPE; see
+  // the level loop): their 200-240 VGPRs leave one 8-wave block per CU, so
+  // the exchange's latency and the per-level barrier are not hidden by a
+  // second block.  FP32 and bf16 blocks fit two per CU, which hides them,
+  // and there the ordering below (exchange within the level) measured
+  // faster (FP32 K=8 2480 vs 2160 G cell-updates/s at 512^3).
+  constexpr bool PIPE = !PROD && sizeof(M) == 8;
   // Edge row handed to the neighbouring wave, double-buffered by the
   // parity of the running level count (one barrier per level suffices).
   __shared__ M edge[2][WY][64];
@@ -942,7 +964,67 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
       // its axis-1 difference after the barrier.
       const int buf = lvl & 1;
       constexpr int er = LO1 ? 0 : R - 1;  // the block-edge row of this wave
-      if constexpr (WY > 1) edge[buf][w][lane] = x[LO1 ? R - 1 : 0];
+      // The neighbouring wave whose edge row this wave's row er reads.  The
+      // block's own upwind-edge wave reads its own slot instead: its row er
+      // is a halo row, where any finite or non-finite value is harmless (no
+      // output cell depends on it), so the read needs no branch.
+      const int wn = LO1 ? (w > 0 ? w - 1 : w) : (w + 1 < WY ? w + 1 : w);
+      if constexpr (WY > 1 && !PIPE) edge[buf][w][lane] = x[LO1 ? R - 1 : 0];
+      if constexpr (PIPE) {
+        // Edge exchange off the critical path.  The row this wave hands to
+        // its neighbour (wr) is never the row that needs the neighbour's
+        // edge (er), so its level-l+1 value is computed first and written
+        // to the other parity's slot at once; the neighbour's level-l edge,
+        // written during the previous level (or with the plane's loads),
+        // was fenced by the previous barrier and is read at the start of
+        // the level; row er is finished last and the level ends with the
+        // barrier.  A slot is rewritten two levels after it was written,
+        // after a barrier that every reader of it had passed.  The block's
+        // upwind-edge wave reads its own slot: its row er is a halo row,
+        // where any value is harmless.  Per cell the operations and their
+        // order are those of a one-step launch.
+        constexpr int wr = LO1 ? R - 1 : 0;
+        M n1 = x[er];  // the block's upwind edge: halo row, any value
+        if constexpr (WY > 1) n1 = edge[buf][wn][lane];
+        M g[R], p2e = M(0);
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          const int r = LO1 ? R - 1 - q : q;  // wr first, er last
+          // The edge lane (63, or 0 with LO2) is a halo column: any value.
+          const M n2 = LO2 ? dpp_shift_any<0x138>(x[r]) : dpp_shift_any<0x130>(x[r]);
+          const M d0 = prev[l][r] - x[r];
+          const M d2 = n2 - x[r];
+          const M p0 = cf.c[0] * d0;
+          const M p2 = cf.c[2] * d2;
+          g[r] = x[r] - p0;
+          if (r != er) {
+            const M d1 = x[LO1 ? r - 1 : r + 1] - x[r];
+            const M p1 = cf.c[1] * d1;
+            g[r] = g[r] - p1;
+            g[r] = g[r] - p2;
+          } else {
+            p2e = p2;
+          }
+          if (WY > 1 && l + 1 < K && r == wr) {
+            edge[buf ^ 1][w][lane] = St<TS>::rnd(g[wr]);  // level l+1 of row wr
+            __builtin_amdgcn_sched_barrier(0);
+          }
+        }
+        const M p1e = cf.c[1] * (n1 - x[er]);
+        g[er] = g[er] - p1e;
+        g[er] = g[er] - p2e;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          prev[l][r] = x[r];
+          x[r] = St<TS>::rnd(g[r]);  // each level as stored
+        }
+        if constexpr (WY > 1) {
+          // The next plane's row wr (its level 0) as late as possible, so
+          // the plane's prefetch has the whole plane to arrive.
+          if (l + 1 == K) edge[buf ^ 1][w][lane] = (m + 1 < cnt) ? nx[wr] : M(0);
+          lds_barrier();
+        }
+      } else {
       M d0[R], d1[R], d2[R];
       M nb2[PROD && sizeof(M) < 8 ? R : 1];  // axis-2 neighbours, for FP32 products
 #pragma unroll
@@ -957,6 +1039,8 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
       M n1 = x[er];  // the block's upwind edge: halo row, any value
       if constexpr (WY > 1) {
         lds_barrier();
+        // (A conditional read here measured faster than the branch-free
+        // one of the PIPE path: FP32 K=8 2480 vs 2040 G cell-updates/s.)
         const M e = edge[buf][LO1 ? (w > 0 ? w - 1 : 0) : (w + 1 < WY ? w + 1 : WY - 1)][lane];
         if (LO1 ? w > 0 : w + 1 < WY) n1 = e;
       }
@@ -1002,6 +1086,7 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
         prev[l][r] = x[r];
         x[r] = St<TS>::rnd(g);  // each level as stored
       }
+      }  // PIPE
       if constexpr (ALL) {
         if (l + 1 < K && m >= K && col_out) {  // level l+1 (level K below)
           TS* ob = (TS*)adj.lv[l] + pw * plane;
@@ -1027,6 +1112,11 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
     }
     pw = pn;
   };
+  if constexpr (WY > 1 && PIPE) {
+    // Level 0 of the first plane's edge row, for the first level's read.
+    edge[0][w][lane] = x[LO1 ? R - 1 : 0];
+    lds_barrier();
+  }
   int64_t m = 0;
   if constexpr (K <= 6 && !PROD) {
     for (; m + 1 < cnt; m += 2) {
