@@ -42,6 +42,7 @@ SIGNATURES = {
     "fdb_upwind_save_vtk": (_i, [_vp, C.c_char_p]),
     "fdb_upwind_save_raw": (_i, [_vp, C.c_char_p]),
     "fdb_upwind_load_raw": (_i, [_vp, C.c_char_p]),
+    "fdb_host_raw_info": (_i, [C.c_char_p, C.POINTER(_i), C.POINTER(_i), _u64p, _u64p, _u64p]),
     "fdb_upwind_ntot": (_i, [_vp, _u64p]),
     "fdb_upwind_dtype": (_i, [_vp, C.POINTER(_i)]),
     "fdb_upwind_set_field": (_i, [_vp, _vp, _i]),

@@ -387,23 +387,66 @@ int fdb_upwind_save_raw(fdb_upwind_t* h, const char* filename) {
   return FDB_OK;
 }
 
+namespace {
+
+// Opens a raw field file and checks its header: the magic, a valid ndims
+// and dtype, axes of at least one cell whose product times the item size
+// fits 63 bits, and a file size of exactly header + payload.  On success
+// the file is positioned at the payload.
+int open_raw(const char* filename, const char* who, RawHeader* hd,
+             std::unique_ptr<FILE, int (*)(FILE*)>* out) {
+  std::unique_ptr<FILE, int (*)(FILE*)> fp(std::fopen(filename, "rb"), &std::fclose);
+  if (!fp) return fail(FDB_ERR_IO, std::string(who) + ": cannot open " + filename);
+  if (std::fread(hd, sizeof(*hd), 1, fp.get()) != 1 ||
+      std::memcmp(hd->magic, kRawMagic, sizeof(kRawMagic)) != 0)
+    return fail(FDB_ERR_IO, std::string(who) + ": not a raw field file");
+  if (hd->ndims < 1 || hd->ndims > 3 || !valid_dtype((int)hd->dtype))
+    return fail(FDB_ERR_IO, std::string(who) + ": bad ndims or dtype in the header");
+  uint64_t ntot = 1;
+  for (int j = 0; j < 3; ++j) {
+    if (hd->n[j] == 0 || (j >= (int)hd->ndims && hd->n[j] != 1) ||
+        ntot > (uint64_t)INT64_MAX / hd->n[j])
+      return fail(FDB_ERR_IO, std::string(who) + ": bad axes in the header");
+    ntot *= hd->n[j];
+  }
+  const size_t esz = fdb::dtype_size((int)hd->dtype);
+  if (ntot > ((uint64_t)INT64_MAX - sizeof(*hd)) / esz)
+    return fail(FDB_ERR_IO, std::string(who) + ": bad axes in the header");
+  // The whole payload must be there (and nothing after it).
+  const uint64_t want = sizeof(*hd) + ntot * esz;
+  if (std::fseek(fp.get(), 0, SEEK_END) != 0 || (uint64_t)std::ftell(fp.get()) != want ||
+      std::fseek(fp.get(), (long)sizeof(*hd), SEEK_SET) != 0)
+    return fail(FDB_ERR_IO, std::string(who) + ": file size does not match its header");
+  *out = std::move(fp);
+  return FDB_OK;
+}
+
+}  // namespace
+
+int fdb_host_raw_info(const char* filename, int* ndims, int* dtype, uint64_t* num_cells,
+                      uint64_t* steps_done, uint64_t* seed) {
+  if (!filename) return fail(FDB_ERR_INVALID, "raw_info: null filename");
+  RawHeader hd{};
+  std::unique_ptr<FILE, int (*)(FILE*)> fp(nullptr, &std::fclose);
+  if (int rc = open_raw(filename, "raw_info", &hd, &fp)) return rc;
+  if (ndims) *ndims = (int)hd.ndims;
+  if (dtype) *dtype = (int)hd.dtype;
+  if (num_cells)
+    for (int j = 0; j < (int)hd.ndims; ++j) num_cells[j] = hd.n[j];
+  if (steps_done) *steps_done = hd.steps_done;
+  if (seed) *seed = hd.seed;
+  return FDB_OK;
+}
+
 int fdb_upwind_load_raw(fdb_upwind_t* h, const char* filename) {
   if (!h || !filename) return fail(FDB_ERR_INVALID, "load_raw: null argument");
-  std::unique_ptr<FILE, int (*)(FILE*)> fp(std::fopen(filename, "rb"), &std::fclose);
-  if (!fp) return fail(FDB_ERR_IO, std::string("load_raw: cannot open ") + filename);
   RawHeader hd{};
-  if (std::fread(&hd, sizeof(hd), 1, fp.get()) != 1 ||
-      std::memcmp(hd.magic, kRawMagic, sizeof(kRawMagic)) != 0)
-    return fail(FDB_ERR_IO, "load_raw: not a raw field file");
+  std::unique_ptr<FILE, int (*)(FILE*)> fp(nullptr, &std::fclose);
+  if (int rc = open_raw(filename, "load_raw", &hd, &fp)) return rc;
   bool same = hd.ndims == (uint32_t)h->ndims && hd.dtype == (uint32_t)h->dtype;
   for (int j = 0; j < 3 && same; ++j) same = hd.n[j] == (j < h->ndims ? h->n[j] : 1);
   if (!same)
     return fail(FDB_ERR_INVALID, "load_raw: the file's shape or storage type differs");
-  // The whole payload must be there (and nothing after it).
-  if (std::fseek(fp.get(), 0, SEEK_END) != 0 ||
-      std::ftell(fp.get()) != (long)(sizeof(hd) + h->ntot * h->esz) ||
-      std::fseek(fp.get(), (long)sizeof(hd), SEEK_SET) != 0)
-    return fail(FDB_ERR_IO, "load_raw: file size does not match its header");
   const uint64_t per = kStageBytes / h->esz;
   char* dst = (char*)h->buf[h->cur];
   for (uint64_t i0 = 0; i0 < h->ntot; i0 += per) {

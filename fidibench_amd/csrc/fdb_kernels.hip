@@ -1390,12 +1390,20 @@ hipError_t rows_dispatch(const Grid3& g, const TS* in, TS* out, const int dirs[3
   Coef<M> cf;
   for (int j = 0; j < 3; ++j) cf.c[j] = (M)g.c[j];
   // Widest access that keeps every row start aligned (f_t included, and
-  // no wider than 16 bytes of f_t).
+  // no wider than 16 bytes of f_t) ...
   int vec = std::min(max_vec<TS>(), max_vec<TF>());
   uintptr_t align_bits = (uintptr_t)in | (uintptr_t)out;
   while (vec > 1 && ((a.n2 % vec) != 0 || (align_bits % (vec * sizeof(TS))) != 0 ||
                      ((uintptr_t)ft % (vec * sizeof(TF))) != 0))
     vec >>= 1;
+  // ... narrowed while that leaves fewer idle lanes in a row's last tile
+  // (a 768-cell bf16 row takes two 512-cell tiles at 8 cells per lane, a
+  // quarter of the lanes idle, but three exact tiles at 4).
+  auto idle = [&](int v) {
+    const int64_t w = 64 * (int64_t)v;
+    return ((a.n2 + w - 1) / w) * w - a.n2;
+  };
+  while (vec > 1 && idle(vec / 2) < idle(vec)) vec >>= 1;
   const int R = cfg.R, WY = cfg.WY;
   a.tiles2 = (a.n2 + 64 * vec - 1) / (64 * vec);
   a.tiles1 = (a.n1 + (int64_t)WY * R - 1) / ((int64_t)WY * R);

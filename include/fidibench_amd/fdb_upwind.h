@@ -105,6 +105,12 @@ int fdb_upwind_save_vtk(fdb_upwind_t* h, const char* filename);
  * shrinks while it is read) leaves the field partly replaced. */
 int fdb_upwind_save_raw(fdb_upwind_t* h, const char* filename);
 int fdb_upwind_load_raw(fdb_upwind_t* h, const char* filename);
+/* The header of a raw field file, checked as load_raw checks it (magic,
+ * ndims 1..3, dtype, axes, exact file size), without a device: ndims,
+ * dtype, num_cells[0..ndims-1], steps done and seed (any may be NULL).
+ * FDB_ERR_IO for a missing, foreign, malformed or truncated file. */
+int fdb_host_raw_info(const char* filename, int* ndims, int* dtype, uint64_t* num_cells,
+                      uint64_t* steps_done, uint64_t* seed);
 
 /* -- extensions of the handle (no member in the reference) -- */
 

@@ -96,6 +96,8 @@ def ref() -> C.CDLL:
         L.ref_upwind_run.restype = C.c_int
         L.ref_upwind_run.argtypes = [C.c_int, _u64p, _dp, _dp, C.c_double, C.c_int, _dp, _dp,
                                      _dp, _dp, C.c_char_p]
+        L.ref_upwind_print.restype = C.c_int
+        L.ref_upwind_print.argtypes = [C.c_int, _u64p, _dp, _dp, C.c_double, C.c_int, _dp]
         L.ref_upwind_time_advect.restype = C.c_double
         L.ref_upwind_time_advect.argtypes = [_u64p, _dp, _dp, C.c_double, C.c_int, _dp, _dp]
         L.ref_upwind_main.restype = C.c_int

@@ -47,9 +47,34 @@ int run(const uint64_t* n, const double* v, const double* len, double dt, int ns
   return 0;
 }
 
+template <size_t ND>
+int run_print(const uint64_t* n, const double* v, const double* len, double dt, int nsteps,
+              const double* f_in) {
+  std::vector<double> vel(v, v + ND), lengths(len, len + ND);
+  std::vector<size_t> cells(n, n + ND);
+  Upwind<ND> up(vel, lengths, cells);
+  if (f_in) std::copy(f_in, f_in + up.ntot, up.f.begin());
+  up.advect(nsteps, dt);
+  up.print();
+  std::cout.flush();
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
+
+// Construct, optionally replace the field, advect, then the reference's
+// own print() (upwind.cxx:105-109) on stdout.
+int ref_upwind_print(int nd, const uint64_t* n, const double* v, const double* len, double dt,
+                     int nsteps, const double* f_in) {
+  switch (nd) {
+    case 1: return run_print<1>(n, v, len, dt, nsteps, f_in);
+    case 2: return run_print<2>(n, v, len, dt, nsteps, f_in);
+    case 3: return run_print<3>(n, v, len, dt, nsteps, f_in);
+    default: return -1;
+  }
+}
 
 // Construct Upwind<nd>(v, len, n); if f_in is non-null, replace the
 // ctor's f[0]=1 field with it; advect(nsteps, dt); copy the field out and

@@ -142,6 +142,59 @@ def cli_cases(manifest):
     manifest["cli"] = out
 
 
+# name, shape, velocity, field ("small:<case>" = that case's f0, "special",
+# "ctor" = the constructor's f[0]=1 field), steps, dt rule
+PRINT = [
+    ("print_c456", (4, 5, 6), (0.7, -0.4, 0.2), "small:c456_mixed", 3, "cfl"),
+    ("print_special", (1, 2, 5), (0.3, 0.2, 0.1), "special", 0, "cfl"),
+    ("print_ctor", (2, 2, 3), (1.0, 1.0, 1.0), "ctor", 2, "main"),
+    ("print_1d", (9,), (-0.6,), "small:c1d_101", 4, "cfl"),
+]
+
+
+def print_field(spec, n):
+    if spec == "ctor":
+        return None
+    if spec == "special":
+        return np.array([0.0, -0.0, 1e-310, 1e300, -1.5e-7, 123456789.0, np.inf, -np.inf,
+                         np.nan, 0.1]).reshape(n)
+    with np.load(os.path.join(HERE, spec.split(":", 1)[1] + ".npz"), allow_pickle=False) as z:
+        return np.ascontiguousarray(z["f0"], dtype=np.float64).ravel()[: int(np.prod(n))] \
+            .reshape(n)
+
+
+def print_cases(manifest):
+    """stdout of the reference's print() (upwind.cxx:105-109), captured in a
+    child process (the harness writes to the C++ std::cout)."""
+    code = ("import sys, json, ctypes as C, numpy as np; sys.path.insert(0, %r); "
+            "from oracle import oracle as O; a = json.loads(sys.argv[1]); "
+            "f = None if a['f'] is None else np.load(a['f'], allow_pickle=False); "
+            "n = a['n']; U = lambda x: (C.c_uint64 * len(x))(*x); D = lambda x: (C.c_double * len(x))(*x); "
+            "p = None if f is None else np.ascontiguousarray(f, dtype=np.float64).ctypes.data_as(C.POINTER(C.c_double)); "
+            "O.ref().ref_upwind_print(len(n), U(n), D(a['v']), D([1.0] * len(n)), a['dt'], a['steps'], p)"
+            ) % ROOT
+    manifest["print"] = {}
+    for name, n, v, spec, steps, rule in PRINT:
+        dt = dt_for(rule, n, v)
+        f0 = print_field(spec, n)
+        fpath = None
+        if f0 is not None:
+            np.savez_compressed(os.path.join(HERE, f"{name}.npz"), f0=f0)
+            fpath = os.path.join(HERE, f"{name}.npz")
+        arg = {"n": list(n), "v": list(v), "dt": dt, "steps": steps,
+               "f": None if fpath is None else fpath.replace(".npz", ".npy")}
+        if fpath is not None:
+            np.save(arg["f"], f0)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        p = subprocess.run([sys.executable, "-c", code, json.dumps(arg)], capture_output=True,
+                           text=True, env=env, timeout=600, check=True)
+        if fpath is not None:
+            os.remove(arg["f"])
+        manifest["print"][name] = dict(shape=list(n), velocity=list(v), dt=repr(dt), steps=steps,
+                                       field=spec, stdout=p.stdout)
+        print("print", name, len(p.stdout.splitlines()), "lines")
+
+
 def large_cases(manifest):
     runs = []
     n = (128, 128, 128)
@@ -176,6 +229,7 @@ def main():
     manifest["vtk"] = {}
     vtk_cases(manifest)
     cli_cases(manifest)
+    print_cases(manifest)
     if "--large" in sys.argv:
         manifest["large"] = {}
         large_cases(manifest)

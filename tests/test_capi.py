@@ -294,3 +294,70 @@ def test_byte_count_overflow_is_invalid_argument():
     assert lib.fdb_filter_create(3, L.u64s([(1 << 31) - 1, (1 << 31) - 1, 1]), L.dbls([0] * 3),
                                  L.dbls([1] * 3), 1, offs, w, C.byref(h)) == L.ERR_INVALID
     assert b"overflows" in lib.fdb_last_error()
+
+
+# ------------------------------------------------------------ raw headers
+
+def _raw_bytes(ndims=3, dtype=0, n=(4, 5, 6), steps=7, seed=99, payload=None, magic=b"FDBFLD01"):
+    import struct
+    head = magic + struct.pack("<II3QQQQ", ndims, dtype, *n, steps, seed, 0)
+    assert len(head) == 64
+    size = {0: 8, 1: 4, 2: 2, 3: 2}.get(dtype, 8)
+    cells = 1
+    for x in n:
+        cells *= x
+    return head + (payload if payload is not None else bytes(cells * size))
+
+
+@pytest.mark.parametrize("kind,expect", [
+    ("good", 0), ("good_bf16sr", 0), ("good_1d", 0), ("truncated", -4), ("longer", -4),
+    ("foreign", -4), ("short_header", -4), ("bad_ndims", -4), ("bad_dtype", -4),
+    ("zero_axis", -4), ("extra_axis", -4), ("huge_axes", -4), ("missing", -4)])
+def test_raw_header_checks_without_a_device(tmp_path, kind, expect):
+    """fdb_host_raw_info applies load_raw's header checks on the host: a
+    missing, foreign, malformed or wrongly sized file is FDB_ERR_IO."""
+    L = _lib()
+    data = {
+        "good": _raw_bytes(),
+        "good_bf16sr": _raw_bytes(dtype=3, n=(3, 2, 1), seed=2**63 + 5),
+        "good_1d": _raw_bytes(ndims=1, dtype=1, n=(10, 1, 1)),
+        "truncated": _raw_bytes()[:-1],
+        "longer": _raw_bytes() + b"\0",
+        "foreign": b"PK\x03\x04" + _raw_bytes()[4:],
+        "short_header": b"FDBFLD01" + bytes(20),
+        "bad_ndims": _raw_bytes(ndims=4),
+        "bad_dtype": _raw_bytes(dtype=7),
+        "zero_axis": _raw_bytes(n=(4, 0, 6), payload=b""),
+        "extra_axis": _raw_bytes(ndims=2, n=(4, 5, 6)),
+        "huge_axes": _raw_bytes(n=(1 << 40, 1 << 30, 1), payload=b""),
+    }.get(kind)
+    path = tmp_path / "f.raw"
+    if data is not None:
+        path.write_bytes(data)
+    nd, dt = C.c_int(-1), C.c_int(-1)
+    n = (C.c_uint64 * 3)()
+    st, sd = C.c_uint64(), C.c_uint64()
+    rc = L.lib().fdb_host_raw_info(str(path).encode(), C.byref(nd), C.byref(dt), n, C.byref(st),
+                                   C.byref(sd))
+    assert rc == expect, (kind, rc, L.lib().fdb_last_error())
+    if kind == "good":
+        assert (nd.value, dt.value, list(n), st.value, sd.value) == (3, 0, [4, 5, 6], 7, 99)
+    if kind == "good_bf16sr":
+        assert (dt.value, sd.value) == (3, 2**63 + 5)
+    if kind == "good_1d":
+        assert (nd.value, n[0]) == (1, 10)
+
+
+@pytest.mark.parametrize("snaps", [1, 2, 3, 5])
+def test_plan_400_steps_few_snapshots(snaps):
+    """The 400-step schedule with few snapshots: valid (every adjoint sees
+    its state, within the budget) and at the closed-form binomial optimum
+    r*T - C(s+r, s+1), s = snaps + 1."""
+    T = 400
+    plan, fw = _plan(T, snaps)
+    assert _simulate(T, snaps, plan) == fw
+    s = snaps + 1
+    r = 0
+    while comb(s + r, s) < T:
+        r += 1
+    assert fw == r * T - comb(s + r, s + 1), (snaps, r, fw)

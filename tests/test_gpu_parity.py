@@ -211,3 +211,25 @@ def test_save_vtk_handle_matches_reference_bytes(manifest, tmp_path, name):
     out = tmp_path / "x.vtk"
     up.saveVTK(str(out))
     assert out.read_bytes() == open(os.path.join(GOLDEN, m["file"]), "rb").read()
+
+
+@pytest.mark.parametrize("name", ["print_c456", "print_special", "print_ctor", "print_1d"])
+def test_print_matches_reference_stdout(manifest, capfd, name):
+    """print() (upwind.cxx:105-109): "i f[i]" lines through the default
+    ostream formatting, byte for byte against the reference's own print()
+    captured by tests/golden/gen_golden.py -- after advect steps, on the
+    constructor's f[0]=1 field, and on -0, a denormal, +-Inf and NaN."""
+    import os
+    from conftest import GOLDEN
+    from fidibench_amd import Upwind
+    m = manifest["print"][name]
+    n = tuple(m["shape"])
+    with Upwind(m["velocity"], (1.0,) * len(n), n) as up:
+        if m["field"] != "ctor":
+            with np.load(os.path.join(GOLDEN, f"{name}.npz"), allow_pickle=False) as z:
+                up.set_field(np.ascontiguousarray(z["f0"], dtype=np.float64))
+        up.advect(m["steps"], float(m["dt"]))
+        capfd.readouterr()
+        up.print()
+        out = capfd.readouterr().out
+    assert out == m["stdout"]

@@ -358,3 +358,23 @@ def test_misfit_gradient_matches_tlm(oracle):
         lhs = float(np.sum((fT - target) * dfT))
         rhs = float(np.sum(gf * df0) + np.dot(gv, dv))
         assert abs(lhs - rhs) <= 1e-12 * (abs(lhs) + 1e-3)
+
+
+# Published SplitMix64 outputs (Steele, Lea & Flood 2014; the reference
+# generator's test vectors): the first outputs for seeds 1234567 and 0.
+SPLITMIX64_KAT = {
+    1234567: [6457827717110365317, 3203168211198807973, 9817491932198370423,
+              4593380528125082431, 16408922859458223821],
+    0: [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F],
+}
+
+
+@pytest.mark.parametrize("seed", sorted(SPLITMIX64_KAT))
+def test_splitmix64_known_answers(oracle, seed):
+    """The workload generator (configs.splitmix64, the oracle's C
+    orc_splitmix64, and through them the device init, which equals them)
+    against the published sequences: draw i is the (i+1)-th output."""
+    from fidibench_amd import configs as K
+    for i, want in enumerate(SPLITMIX64_KAT[seed]):
+        assert K.splitmix64(seed, i) == want, (seed, i)
+        assert oracle.lib().orc_splitmix64(seed, i) == want, (seed, i)
