@@ -185,6 +185,21 @@ int step_into(fdb_tape* t, void* dst, hipStream_t s) {
 // A scratch buffer that is not the working state.
 void* free_scratch(fdb_tape* t) { return t->w == t->scratch[0] ? t->scratch[1] : t->scratch[0]; }
 
+// Steps of the next fused pass over `left` remaining steps (1 = a one-step
+// launch).  Each storage type has a fastest depth per step at 512^3
+// (FP64 and bf16: 6, FP32: 8); passes use it while at least two such
+// passes remain, and a remainder of at most 8 steps goes into the deepest
+// single pass that fits.
+int pass_steps(int dtype, const fdb::Grid3& g, int left, int cap) {
+  const int pref = dtype == FDB_F32 ? 8 : 6;
+  if (left >= 2 * pref || left > fdb::kMaxTimeBlock) {
+    if (pref <= cap && fdb::fused_pass_supported(dtype, g, pref)) return pref;
+  }
+  for (int K : {8, 7, 6, 4, 3, 2})
+    if (K <= left && K <= cap && fdb::fused_pass_supported(dtype, g, K)) return K;
+  return 1;
+}
+
 // Advance the working state by a steps; the last one lands in `last_dst`
 // when given (the slot of a snapshot about to be stored).  The states in
 // between are not kept, so runs of steps go through fused multi-step
@@ -194,17 +209,17 @@ int advance(fdb_tape* t, int a, void* last_dst, hipStream_t s) {
   int left = a;
   while (left > 0) {
     int done = 0;
-    for (int k : {8, 6, 4, 3, 2}) {
-      if (k > left || k > t->time_block) continue;
+    const int k = pass_steps(t->f_dtype, t->g, left, t->time_block);
+    if (k > 1) {
       void* dst = (k == left && last_dst) ? last_dst : free_scratch(t);
       const hipError_t e = fdb::launch_fused_pass(t->f_dtype, t->g, t->w, dst, k, s);
-      if (e == hipErrorNotSupported) continue;
-      TAPE_HIP(e);
-      t->w = dst;
-      t->wstep += k;
-      t->fwd_steps += k;
-      done = k;
-      break;
+      if (e != hipErrorNotSupported) {
+        TAPE_HIP(e);
+        t->w = dst;
+        t->wstep += k;
+        t->fwd_steps += k;
+        done = k;
+      }
     }
     if (done == 0) {
       void* dst = (left == 1 && last_dst) ? last_dst : free_scratch(t);
@@ -400,12 +415,7 @@ int backward_shift(fdb_tape* t, const void* lam_T, void* lam0, double* grad_v, h
   // Pass lengths for the T-1 plain steps: the deepest fused depth that fits.
   std::vector<int> ks;
   for (int left = t->nsteps - 1; left > 0;) {
-    int k = 1;
-    for (int K : {8, 7, 6, 4, 3, 2})
-      if (K <= left && K <= t->time_block && fdb::fused_pass_supported(ld, ga, K)) {
-        k = K;
-        break;
-      }
+    const int k = pass_steps(ld, ga, left, t->time_block);
     ks.push_back(k);
     left -= k;
   }

@@ -43,3 +43,14 @@ def test_committed_traffic_parses(key, summary, compulsory):
 
 def test_unknown_traffic_key_is_none():
     assert _bench().pmc_traffic("no_such_kernel") is None
+
+
+@pytest.mark.parametrize("key,k", [("f64_k6", 6), ("f64_k8", 8), ("f32_k8", 8), ("bf16_k6", 6)])
+def test_committed_valu_counts_parse(key, k):
+    """SQ_INSTS_VALU per launch of the fused kernels: 512^3 cells x k
+    levels, 9-12 VALU instructions per 64-lane cell-level once the tile
+    halo (about 1.2-1.3x) is counted, so 10-20 per output cell-update."""
+    v = _bench().pmc_traffic("valu_" + key)
+    assert v is not None
+    per = v * 64 / (512 ** 3 * k)
+    assert 10 <= per <= 20, per
