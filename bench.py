@@ -438,7 +438,7 @@ def large_adjoint(shape=(1024, 1024, 768), steps=400):
     lam64, gv64 = g64.pop("grad_f0"), g64.pop("grad_v")
     fwd = {}
     for name, code, tb in (("plain_bf16_forward_s", L.BF16, 8),
-                           ("plain_bf16sr_forward_s", L.BF16_SR, 1)):
+                           ("plain_bf16sr_forward_s", L.BF16_SR, 8)):
         f0 = init_field(w, L.BF16)
         sync()
         gpu_busy()

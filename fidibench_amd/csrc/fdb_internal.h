@@ -72,11 +72,14 @@ hipError_t launch_advect_pass(int dtype, const Grid3& g, void* const bufs[2], in
 
 // One launch of k >= 2 fused steps from src into dst (distinct buffers);
 // src is not written.  With levels != nullptr the states after steps
-// 1..k-1 are also written, to levels[0..k-2] (FP64, k = 4 or 8).
-// hipErrorNotSupported when no single launch does this for the storage
-// type and shape (the caller then steps one by one).
+// 1..k-1 are also written, to levels[0..k-2] (FP64, k = 4 or 8).  For
+// FDB_BF16_SR the steps are number step0 .. step0+k-1 of a run seeded
+// `seed` (their stochastic-rounding keys).  hipErrorNotSupported when no
+// single launch does this for the storage type and shape (the caller then
+// steps one by one).
 hipError_t launch_fused_pass(int dtype, const Grid3& g, const void* src, void* dst, int k,
-                             hipStream_t s, void* const* levels = nullptr);
+                             hipStream_t s, void* const* levels = nullptr, uint64_t step0 = 0,
+                             uint64_t seed = 0);
 
 // Whether launch_fused_pass would run k fused steps of this storage type
 // (FDB_F64, FDB_F32, FDB_BF16) and shape in one launch.

@@ -789,6 +789,7 @@ struct TbArgs {
   int u0;
   int64_t chunk;               // output planes per tile along axis 0
   int64_t tiles2, tiles1, ntiles;
+  uint32_t sr_key[kMaxTimeBlock];  // kTbSR: stochastic-rounding key of level l's step
 };
 
 // Modes of the temporally blocked kernel.
@@ -804,7 +805,15 @@ struct TbArgs {
 //                the one-step adjoint's velocity products -- in FP64
 //                (differences taken in FP64 as there), Neumaier-compensated
 //                per thread and reduced per block into aux.partials[block*6].
-constexpr int kTbPlain = 0, kTbStoreAll = 1, kTbAdjoint = 2;
+//   kTbSR        K forward steps in bf16 storage, each level rounded
+//                stochastically with the bits a one-step launch of that step
+//                draws (lowbias32 of the cell's pair index and the step's
+//                key, low half for an even cell, high half for an odd one),
+//                so the result is the one-step launches' bit for bit.  The
+//                two lanes of a cell pair (even row length, even K: lane
+//                parity = column parity) split the hashing: each hashes half
+//                the rows and they swap results with a DPP quad permutation.
+constexpr int kTbPlain = 0, kTbStoreAll = 1, kTbAdjoint = 2, kTbSR = 3;
 struct TbAux {
   const void* fs[kMaxTimeBlock];
   void* lv[kMaxTimeBlock];
@@ -851,7 +860,8 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
                                                      TS* __restrict__ out, TbArgs a,
                                                      Coef<typename St<TS>::M> cf, TbAux adj) {
   using M = typename St<TS>::M;
-  constexpr bool PROD = MODE == kTbAdjoint, ALL = MODE == kTbStoreAll;
+  constexpr bool PROD = MODE == kTbAdjoint, ALL = MODE == kTbStoreAll, SR = MODE == kTbSR;
+  static_assert(!SR || sizeof(TS) == 2, "stochastic rounding is for bf16 storage");
   // FP64 forward passes exchange the edge rows one level ahead (PIPE; see
   // the level loop): their 200-240 VGPRs leave one 8-wave block per CU, so
   // the exchange's latency and the per-level barrier are not hidden by a
@@ -892,6 +902,11 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
     roff[r] = pmod(rl, a.n1) * a.n2;
     row_out[r] = (LO1 ? br >= K : br < WY * R - K) && rl >= 0 && rl < a.n1;
   }
+  // SR: this lane hashes rows 2j (even lane) or 2j+1 (odd lane) of its
+  // column's pairs; sword[j] is that cell's pair word for the current plane.
+  constexpr int J = (R + 1) / 2;
+  const bool odd = (lane & 1) != 0;
+  uint32_t sword[SR ? J : 1];
   // Planes: output p_lo .. p_hi-1; march K planes earlier (upwind side).
   const int64_t plane = a.n1 * a.n2;
   const int64_t p_lo = ch * a.chunk;
@@ -948,6 +963,15 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
   auto plane_step = [&](const int64_t m) __attribute__((always_inline)) {
     int64_t pn = pw + step;
     pn = pn < 0 ? a.n0 - 1 : (pn == a.n0 ? 0 : pn);
+    if constexpr (SR) {
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        // (row offsets are uniform: the lane's row is picked per plane)
+        const int64_t ro = odd ? roff[2 * j + 1 < R ? 2 * j + 1 : R - 1] : roff[2 * j];
+        const uint64_t pair = ((uint64_t)(pw * plane + ro) + col) >> 1;
+        sword[j] = (uint32_t)(pair ^ (pair >> 32));
+      }
+    }
     if (m + 1 < cnt) {
       const TS* base = in + pn * plane;
 #pragma unroll
@@ -1077,6 +1101,7 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
         d1[r] = cf.c[1] * d1[r];
         d2[r] = cf.c[2] * d2[r];
       }
+      uint32_t own = 0, par = 0;  // SR: the hashes of rows 2j, 2j+1 (this lane's, its partner's)
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         M g = x[r];
@@ -1084,7 +1109,22 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
         g = g - d1[r];
         g = g - d2[r];
         prev[l][r] = x[r];
-        x[r] = St<TS>::rnd(g);  // each level as stored
+        if constexpr (SR) {
+          // The pair's hash (computed by the lane of the row's parity, at
+          // the row pair's first row, so only two are live); the even cell
+          // takes its low half, the odd cell its high half, which
+          // v_cvt_sr_bf16_f32 reads from bits 31:16 of its operand.
+          if (r % 2 == 0) {
+            own = lowbias32(sword[r / 2] ^ a.sr_key[l]);
+            par = __builtin_amdgcn_mov_dpp(own, 0xb1, 0xf, 0xf, true);
+          }
+          const uint32_t h = (((r & 1) != 0) == odd) ? own : par;
+          const s16x2 z = {0, 0};
+          x[r] = __builtin_bit_cast(
+              float, __builtin_amdgcn_cvt_sr_bf16_f32(z, g, odd ? h : h << 16, true));
+        } else {
+          x[r] = St<TS>::rnd(g);  // each level as stored
+        }
       }
       }  // PIPE
       if constexpr (ALL) {
@@ -1474,11 +1514,23 @@ TbCfg tb_cfg(int bytes, int K) { return tuning().tb[bytes][K]; }
 // levels != nullptr: also store levels 1..K-1 to levels[0..K-2] (K = 4
 // or 8, FP64 only: with one 4-byte cell per lane the FP32 kernel writes its
 // 8 levels at ~2 TB/s, slower than one-step launches storing 16 B per lane).
+// Launch shape of the stochastic-rounding bf16 pass (K = 2, 4 or 8: even,
+// so lane pairs hold cell pairs): within 128 VGPRs, i.e. four waves per
+// SIMD like the round-to-nearest pass -- at K = 8 with 8 rows per wave
+// (9 need 132 VGPRs).
+TbCfg tb_sr_cfg(int K) { return K == 8 ? TbCfg{8, 8, 0} : TbCfg{9, 4, 0}; }
+
+bool tb_sr_ok(const Grid3& g, int K) {
+  return (K == 2 || K == 4 || K == 8) && g.n[2] % 2 == 0;
+}
+
 template <typename TS>
 hipError_t tb_dispatch(const Grid3& g, const TS* in, TS* out, int K, hipStream_t s,
-                       void* const* levels = nullptr) {
+                       void* const* levels = nullptr, const uint32_t* sr_keys = nullptr) {
   using M = typename St<TS>::M;
-  const TbCfg cfg = tb_cfg((int)sizeof(TS), K);
+  const bool sr = sr_keys != nullptr;
+  if (sr && (sizeof(TS) != 2 || !tb_sr_ok(g, K))) return hipErrorNotSupported;
+  const TbCfg cfg = sr ? tb_sr_cfg(K) : tb_cfg((int)sizeof(TS), K);
   const int R = cfg.R, WY = cfg.WY;
   if (!(g.act[0] && g.act[1] && g.act[2]) || WY * R <= K) return hipErrorNotSupported;
   TbArgs a;
@@ -1499,10 +1551,12 @@ hipError_t tb_dispatch(const Grid3& g, const TS* in, TS* out, int K, hipStream_t
   const bool all = levels != nullptr;
   if (all && sizeof(TS) != 8) return hipErrorNotSupported;
   for (int l = 0; all && l + 1 < K; ++l) aux.lv[l] = levels[l];
+  for (int l = 0; sr && l < K; ++l) a.sr_key[l] = sr_keys[l];
+  const int mode = all ? kTbStoreAll : (sr ? kTbSR : kTbPlain);
   bool launched = false;
 #define FDB_TB_DIR(RR, KK, W, L1, L2, MD)                                                   \
   if (!launched && R == RR && K == KK && WY == W && lo1 == L1 && lo2 == L2 &&               \
-      all == (MD == kTbStoreAll)) {                                                         \
+      mode == MD) {                                                                         \
     auto kern = tb_kernel<TS, RR, KK, W, L1, L2, MD>;                                       \
     int64_t nch;                                                                            \
     if (cfg.blocks > 0) {                                                                   \
@@ -1530,6 +1584,11 @@ hipError_t tb_dispatch(const Grid3& g, const TS* in, TS* out, int K, hipStream_t
   if constexpr (sizeof(TS) == 8) {
     FDB_TB(9, 4, 4, kTbStoreAll)
     FDB_TB(9, 8, 8, kTbStoreAll)
+  }
+  if constexpr (sizeof(TS) == 2) {
+    FDB_TB(9, 2, 4, kTbSR)
+    FDB_TB(9, 4, 4, kTbSR)
+    FDB_TB(8, 8, 8, kTbSR)
   }
 #undef FDB_TB
 #undef FDB_TB_DIR
@@ -1666,10 +1725,13 @@ bool make_grid(int ndims, const uint64_t* n, const double* v, const double* len,
 
 bool fused_pass_supported(int dtype, const Grid3& g, int k) {
   if (k < 2 || k > kMaxTimeBlock) return false;
-  if (dtype != FDB_F64 && dtype != FDB_F32 && dtype != FDB_BF16) return false;
+  if (dtype != FDB_F64 && dtype != FDB_F32 && dtype != FDB_BF16 && dtype != FDB_BF16_SR)
+    return false;
   const int bytes = (int)dtype_size(dtype);
-  const TbCfg c = tb_cfg(bytes, k);
-  return tb_shape_ok(c, k) && g.act[0] && g.act[1] && g.act[2] && c.WY * c.R > k &&
+  const bool sr = dtype == FDB_BF16_SR;
+  if (sr && !tb_sr_ok(g, k)) return false;
+  const TbCfg c = sr ? tb_sr_cfg(k) : tb_cfg(bytes, k);
+  return (sr || tb_shape_ok(c, k)) && g.act[0] && g.act[1] && g.act[2] && c.WY * c.R > k &&
          g.n[2] * (int64_t)bytes <= INT32_MAX;
 }
 
@@ -1678,21 +1740,28 @@ const char* tuning_error() {
   return t.error.empty() ? nullptr : t.error.c_str();
 }
 
+uint32_t sr_step_key(uint64_t seed, uint64_t step) {
+  const uint32_t k1 = lowbias32((uint32_t)step + 0x9E3779B9u * (uint32_t)(step >> 32));
+  return lowbias32(k1 ^ (uint32_t)seed ^ ((uint32_t)(seed >> 32) * 0x85EBCA6Bu));
+}
+
 hipError_t launch_fused_pass(int dtype, const Grid3& g, const void* src, void* dst, int k,
-                             hipStream_t s, void* const* levels) {
+                             hipStream_t s, void* const* levels, uint64_t step0, uint64_t seed) {
   if (k < 2 || src == dst) return hipErrorNotSupported;
   switch (dtype) {
     case FDB_F64: return tb_dispatch<double>(g, (const double*)src, (double*)dst, k, s, levels);
     case FDB_F32: return tb_dispatch<float>(g, (const float*)src, (float*)dst, k, s, levels);
     case FDB_BF16:
       return tb_dispatch<uint16_t>(g, (const uint16_t*)src, (uint16_t*)dst, k, s, levels);
+    case FDB_BF16_SR: {
+      if (levels || k > kMaxTimeBlock) return hipErrorNotSupported;
+      uint32_t keys[kMaxTimeBlock];
+      for (int l = 0; l < k; ++l) keys[l] = sr_step_key(seed, step0 + (uint64_t)l);
+      return tb_dispatch<uint16_t>(g, (const uint16_t*)src, (uint16_t*)dst, k, s, nullptr,
+                                   keys);
+    }
     default: return hipErrorNotSupported;
   }
-}
-
-uint32_t sr_step_key(uint64_t seed, uint64_t step) {
-  const uint32_t k1 = lowbias32((uint32_t)step + 0x9E3779B9u * (uint32_t)(step >> 32));
-  return lowbias32(k1 ^ (uint32_t)seed ^ ((uint32_t)(seed >> 32) * 0x85EBCA6Bu));
 }
 
 hipError_t launch_advect_pass(int dtype, const Grid3& g, void* const bufs[2], int* cur, int k,
@@ -1714,8 +1783,10 @@ hipError_t launch_advect_pass(int dtype, const Grid3& g, void* const bufs[2], in
         case FDB_BF16:
           err = tb_dispatch<uint16_t>(g, (const uint16_t*)src, (uint16_t*)dst, K, s);
           break;
-        // Stochastic rounding draws its bits per (step, cell): one step per launch.
-        case FDB_BF16_SR: err = hipErrorNotSupported; break;
+        // Stochastic rounding: the bits of step stepno + l at level l.
+        case FDB_BF16_SR:
+          err = launch_fused_pass(dtype, g, src, dst, K, s, nullptr, stepno, seed);
+          break;
         default: return hipErrorInvalidValue;
       }
       if (err == hipSuccess) {

@@ -187,11 +187,11 @@ void* free_scratch(fdb_tape* t) { return t->w == t->scratch[0] ? t->scratch[1] :
 
 // Steps of the next fused pass over `left` remaining steps (1 = a one-step
 // launch).  Each storage type has a fastest depth per step at 512^3
-// (FP64 and bf16: 6, FP32: 8); passes use it while at least two such
-// passes remain, and a remainder of at most 8 steps goes into the deepest
-// single pass that fits.
+// (FP64 and bf16: 6, FP32 and bf16 with stochastic rounding: 8); passes use
+// it while at least two such passes remain, and a remainder of at most 8
+// steps goes into the deepest single pass that fits.
 int pass_steps(int dtype, const fdb::Grid3& g, int left, int cap) {
-  const int pref = dtype == FDB_F32 ? 8 : 6;
+  const int pref = (dtype == FDB_F32 || dtype == FDB_BF16_SR) ? 8 : 6;
   if (left >= 2 * pref || left > fdb::kMaxTimeBlock) {
     if (pref <= cap && fdb::fused_pass_supported(dtype, g, pref)) return pref;
   }
@@ -212,7 +212,10 @@ int advance(fdb_tape* t, int a, void* last_dst, hipStream_t s) {
     const int k = pass_steps(t->f_dtype, t->g, left, t->time_block);
     if (k > 1) {
       void* dst = (k == left && last_dst) ? last_dst : free_scratch(t);
-      const hipError_t e = fdb::launch_fused_pass(t->f_dtype, t->g, t->w, dst, k, s);
+      // Stochastic rounding is keyed by the absolute step (recomputed
+      // states get the bits the forward drew).
+      const hipError_t e = fdb::launch_fused_pass(t->f_dtype, t->g, t->w, dst, k, s, nullptr,
+                                                  (uint64_t)t->wstep, 0);
       if (e != hipErrorNotSupported) {
         TAPE_HIP(e);
         t->w = dst;

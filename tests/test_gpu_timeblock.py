@@ -174,3 +174,31 @@ def test_device_pointer_bf16_sr_continues_one_run(oracle, time_block):
     torch.cuda.synchronize()
     got0 = (b if res.value else a).cpu().numpy().view(np.uint16)
     assert np.array_equal(got0, ref0)
+
+
+@pytest.mark.parametrize("shape", [(9, 40, 100), (13, 25, 64), (6, 17, 130), (5, 48, 67),
+                                   (4, 21, 2)])
+@pytest.mark.parametrize("signs", list(itertools.product((1, -1), repeat=3)))
+def test_time_blocked_bf16_sr_bit_exact(oracle, shape, signs):
+    """Stochastically rounded bf16 in fused passes (K = 2, 4, 8 with an even
+    row length): every level draws the bits of its absolute step, so any
+    time-blocking depth equals the oracle's one-step restatement bit for
+    bit; odd row lengths take one-step launches."""
+    from fidibench_amd import Upwind, configs as K
+    v = tuple(s * m for s, m in zip(signs, (0.3, 0.45, 0.2)))
+    dt = K.cfl_dt(shape, v)
+    f0 = oracle.uniform(shape, 0.0, 1.0, seed=sum(shape) + 23)
+    seed = 0xDEADBEEF12345 + sum(shape)
+    for k in (1, 2, 4, 6, 8):
+        up = Upwind(v, (1, 1, 1), shape, dtype="bf16sr")
+        up.set_seed(seed)
+        up.set_field(f0)
+        start = up.get_field()
+        ref = oracle.advect_bf16_sr(start, v, (1, 1, 1), dt, 11, seed=seed, step0=0)
+        up.set_time_block(k)
+        ms, launches = up.advect_timed(5, dt)  # a run in two calls: steps 0-4, 5-10
+        ms, launches2 = up.advect_timed(6, dt)
+        assert np.array_equal(bits(up.get_field()), bits(ref)), (shape, signs, k)
+        if k >= 2 and shape[2] % 2 == 0:
+            assert launches + launches2 < 11  # fused passes ran
+        up.close()
