@@ -75,8 +75,10 @@ def parse():
     p.add_argument("--n", type=int, default=512, help="cells per axis")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--only", default="", help="comma list of: " + ",".join(ALL))
-    p.add_argument("--time-blocks", default="1,4,6,8",
-                   help="comma list of time-blocking depths k (steps fused per launch)")
+    p.add_argument("--time-blocks", default="1,4,6,7,8",
+                   help="comma list of time-blocking depths k (steps fused per launch); "
+                        "value is the best one's (k=6 at 100 steps; k=7 at 20, where 7+7+6 "
+                        "beats 6+6+6+2)")
     return p.parse_args()
 
 
