@@ -907,6 +907,10 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
   constexpr int J = (R + 1) / 2;
   const bool odd = (lane & 1) != 0;
   uint32_t sword[SR ? J : 1];
+  // SR: v_perm_b32 selectors (own hash = bytes 7..4, partner's = 3..0) that
+  // put the cell's 16 rounding bits in bits 31:16 (low bytes zero), for an
+  // even row (the even lane's own hash) and an odd row (the odd lane's own).
+  const uint32_t ssel[2] = {odd ? 0x03020c0cu : 0x05040c0cu, odd ? 0x07060c0cu : 0x01000c0cu};
   // Planes: output p_lo .. p_hi-1; march K planes earlier (upwind side).
   const int64_t plane = a.n1 * a.n2;
   const int64_t p_lo = ch * a.chunk;
@@ -928,7 +932,11 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
     for (int r = 0; r < R; ++r) x[r] = St<TS>::ld(buf_ld(row_rsrc(base + roff[r], rbytes), cb, in));
 #pragma unroll
     for (int r = 0; r < R; ++r) settle(x[r]);
+#pragma unroll
+    for (int r = 0; r < R; ++r) nx[r] = x[r];  // defined where a prefetch is skipped
   }
+  constexpr bool kCondPrefetch = PIPE && K > 6;
+  constexpr bool kWaitBeforeStores = (MODE == kTbPlain || MODE == kTbSR) && !kCondPrefetch;
   // PROD: fv[l] holds the state of level l at the current plane; after
   // level l it is reloaded with the next plane's (always a valid plane, so
   // every load is unconditional).  The states are needed at output cells
@@ -972,8 +980,13 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
         sword[j] = (uint32_t)(pair ^ (pair >> 32));
       }
     }
-    if (m + 1 < cnt) {
-      const TS* base = in + pn * plane;
+    if (!kCondPrefetch || m + 1 < cnt) {
+      // The next plane.  Loaded unconditionally (after the last plane the
+      // current one again, from L2, unused) where registers allow: a
+      // prefetch under a branch makes the compiler merge nx across the
+      // branch right after the loads, which waits for them there.  FP64 at
+      // K > 6 keeps the branch (unconditional loads there spill).
+      const TS* base = in + (m + 1 < cnt ? pn : pw) * plane;
 #pragma unroll
       for (int r = 0; r < R; ++r) nx[r] = St<TS>::ld(buf_ld(row_rsrc(base + roff[r], rbytes), cb, in));
     }
@@ -1045,7 +1058,7 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
         if constexpr (WY > 1) {
           // The next plane's row wr (its level 0) as late as possible, so
           // the plane's prefetch has the whole plane to arrive.
-          if (l + 1 == K) edge[buf ^ 1][w][lane] = (m + 1 < cnt) ? nx[wr] : M(0);
+          if (l + 1 == K) edge[buf ^ 1][w][lane] = nx[wr];
           lds_barrier();
         }
       } else {
@@ -1108,20 +1121,24 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
         g = g - d0[r];
         g = g - d1[r];
         g = g - d2[r];
+        const M gone = prev[l][r];  // SR: a bf16 value, so its low 16 bits are zero
         prev[l][r] = x[r];
         if constexpr (SR) {
           // The pair's hash (computed by the lane of the row's parity, at
           // the row pair's first row, so only two are live); the even cell
           // takes its low half, the odd cell its high half, which
-          // v_cvt_sr_bf16_f32 reads from bits 31:16 of its operand.
+          // v_cvt_sr_bf16_f32 reads from bits 31:16 of its operand: one
+          // v_perm_b32 puts the right half of the right hash there.  The
+          // conversion writes the high half of a register whose low half is
+          // kept, so it is given a dead bf16 value (low half zero).
           if (r % 2 == 0) {
             own = lowbias32(sword[r / 2] ^ a.sr_key[l]);
             par = __builtin_amdgcn_mov_dpp(own, 0xb1, 0xf, 0xf, true);
           }
-          const uint32_t h = (((r & 1) != 0) == odd) ? own : par;
-          const s16x2 z = {0, 0};
+          const uint32_t rbits = __builtin_amdgcn_perm(own, par, ssel[r & 1]);
           x[r] = __builtin_bit_cast(
-              float, __builtin_amdgcn_cvt_sr_bf16_f32(z, g, odd ? h : h << 16, true));
+              float, __builtin_amdgcn_cvt_sr_bf16_f32(__builtin_bit_cast(s16x2, gone), g,
+                                                      rbits, true));
         } else {
           x[r] = St<TS>::rnd(g);  // each level as stored
         }
@@ -1140,16 +1157,21 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
 #pragma unroll
       for (int j = 0; j < 3; ++j) acc.add(j, ps[j]);
     }
+    // The next plane's loads, issued at the top of this plane, are waited
+    // for here, before this plane's stores: the memory counter retires in
+    // issue order, so a wait placed after the (row-conditional) stores would
+    // also wait for their write acknowledgements, in every wave at once.
+    // (The store-all and adjoint modes issue memory operations within the
+    // level loop, which a full wait here would catch.)
+    if constexpr (kWaitBeforeStores) __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
     if (m >= K && col_out) {
       TS* ob = out + pw * plane;
 #pragma unroll
       for (int r = 0; r < R; ++r)
         if (row_out[r]) buf_st_nt(St<TS>::st(x[r]), row_rsrc(ob + roff[r], rbytes), cb);
     }
-    if (m + 1 < cnt) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) x[r] = nx[r];
-    }
+    for (int r = 0; r < R; ++r) x[r] = nx[r];
     pw = pn;
   };
   if constexpr (WY > 1 && PIPE) {
