@@ -18,9 +18,11 @@ idle host work between legs, which would cost the next 100-step run
 12-15 %.
 
 One JSON line: value = FP64 cell-updates/s at the best time-blocking depth
-k (steps fused per launch), plus variants:
-  {f64,f32,bf16}_k{k}  the workload per storage type and depth; every k's
-                   final field is compared bit for bit with k = 1's
+k (steps fused per launch; k = 0: the library's automatic least-time split
+of the run into passes), plus variants:
+  {f64,f32,bf16}_k{k}  the workload per storage type and depth (_auto for
+                   k = 0); every k's final field is compared bit for bit
+                   with k = 1's
   mixed_gauss1024  1024^3 x 200 Gaussian bubble in FP64/FP32/BF16 (round to
                    nearest and stochastic): rates, max-abs / RMS / max-rel
                    error vs FP64, mass drift
@@ -75,10 +77,9 @@ def parse():
     p.add_argument("--n", type=int, default=512, help="cells per axis")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--only", default="", help="comma list of: " + ",".join(ALL))
-    p.add_argument("--time-blocks", default="1,4,6,7,8",
-                   help="comma list of time-blocking depths k (steps fused per launch); "
-                        "value is the best one's (k=6 at 100 steps; k=7 at 20, where 7+7+6 "
-                        "beats 6+6+6+2)")
+    p.add_argument("--time-blocks", default="1,4,6,7,8,0",
+                   help="comma list of time-blocking depths k (steps fused per launch; 0 = "
+                        "the library's automatic split); value is the best one's")
     return p.parse_args()
 
 
@@ -534,9 +535,20 @@ def fp64_checksum(w, steps, k):
     return cs
 
 
+def klabel(dt_name, k):
+    return f"{dt_name}_auto" if k == 0 else f"{dt_name}_k{k}"
+
+
+def plan(w, dtype, steps, k):
+    """Steps per launch of the run (fdb_plan_passes)."""
+    from fidibench_amd import _lib as L
+    return L.plan_passes(w.num_cells, w.velocity, w.lengths, w.dt, L.DTYPES[dtype], steps, k)
+
+
 def roofline(w, dtype, k, steps):
     """Roofline of the dominant kernel: the k-step forward kernel of
-    `dtype` (k = 1: the one-step kernel).  Its average launch duration is
+    `dtype` (k = 1: the one-step kernel; k = 0, the automatic split: the
+    depth that advances the most steps of the run).  Its average launch duration is
     measured here (kernel_launch_ms).  frac is a position on the HBM
     roofline: compulsory bytes per launch (read state t, write t+k: 2*s per
     cell) / duration / 8 TB/s.  effective_frac credits 2*s bytes to every
@@ -546,6 +558,9 @@ def roofline(w, dtype, k, steps):
     lane-operations issued per second (SQ_INSTS_VALU x 64 per launch, from
     the committed summary) against the 39.3 T the clock allows."""
     s = BYTES[dtype]
+    schedule = plan(w, dtype, steps, k)
+    if k == 0:
+        k = max(set(schedule), key=lambda d: (d * schedule.count(d), d))
     launch_ms, _ = kernel_launch_ms(w, dtype, k)
     comp = 2.0 * s * w.ntot
     achieved = comp / (launch_ms * 1e-3)
@@ -554,7 +569,7 @@ def roofline(w, dtype, k, steps):
            "kernel": (f"tb_kernel<{ {'f64': 'double', 'f32': 'float'}.get(dtype, 'bf16')}> "
                       f"({dtype.upper()} forward, {k} fused steps per launch)" if k > 1 else
                       f"rows_kernel ({dtype.upper()} forward, 1 step per launch)"),
-           "steps_per_launch": k, "avg_launch_ms": launch_ms,
+           "steps_per_launch": k, "avg_launch_ms": launch_ms, "schedule": schedule,
            "algorithmic_bytes_per_launch": comp,
            "note": ("algorithmic bytes = compulsory bytes per launch: read state t and write "
                     "state t+k once (2*s per cell); effective_* credit them to each of the k "
@@ -601,7 +616,7 @@ def main():
         for k in ks:
             ms, launches, up = run_forward(w, dt_name, a.steps, a.warmup, k)
             cu = w.ntot * a.steps / (ms * 1e-3)
-            res[f"{dt_name}_k{k}"] = {"cell_updates_per_s": cu, "ms": ms, "launches": launches,
+            res[klabel(dt_name, k)] = {"cell_updates_per_s": cu, "ms": ms, "launches": launches,
                                       "time_block": k,
                                       "hbm_frac_compulsory": cu * 2 * BYTES[dt_name] / HBM_PEAK}
             if dt_name not in best_of or cu > best_of[dt_name][0]:
@@ -610,7 +625,7 @@ def main():
                 ref_up = up
             else:
                 same[k] = same_bits(up, ref_up)
-                res[f"{dt_name}_k{k}"]["bit_identical_to_k1"] = same[k]
+                res[klabel(dt_name, k)]["bit_identical_to_k1"] = same[k]
                 up.close()
         ref_up.close()
         parity["checks"][f"{dt_name}_every_k_equals_k1"] = all(same.values())

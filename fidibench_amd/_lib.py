@@ -18,6 +18,7 @@ ITEMSIZE = {F64: 8, F32: 4, BF16: 2, BF16_SR: 2}
 
 INIT_DELTA, INIT_SPHERE, INIT_GAUSS, INIT_GAUSS_SUM, INIT_UNIFORM = range(5)
 CK_ADVANCE, CK_STORE, CK_RESTORE, CK_POP, CK_ADJOINT = 1, 2, 3, 4, 5
+TIME_BLOCK_AUTO = 0  # time_block: the library's least-time split of each advect
 GRAD_CHECKPOINT, GRAD_SHIFT = 0, 1
 GRAD_METHODS = {"checkpoint": GRAD_CHECKPOINT, "shift": GRAD_SHIFT}
 
@@ -57,6 +58,8 @@ SIGNATURES = {
                             C.POINTER(_i)]),
     "fdb_advect_dev_seeded": (_i, [_i, _u64p, _dp, _dp, C.c_double, _i, _vp, _vp, _i, _i,
                                    C.c_uint64, C.c_uint64, _vp, C.POINTER(_i)]),
+    "fdb_plan_passes": (_i, [_i, _u64p, _dp, _dp, C.c_double, _i, _i, _i, C.POINTER(_i), _i,
+                             C.POINTER(_i)]),
     "fdb_adjoint_step_dev": (_i, [_i, _u64p, _dp, _dp, C.c_double, _i, _vp, _vp, _i, _vp, _vp,
                                   _vp]),
     "fdb_adjoint_step_dev_noalloc": (_i, [_i, _u64p, _dp, _dp, C.c_double, _i, _vp, _vp, _i,
@@ -135,3 +138,18 @@ def u64s(a):
 
 def dbls(a):
     return (C.c_double * max(1, len(a)))(*[float(x) for x in a])
+
+
+def plan_passes(num_cells, velocity, lengths, dt, dtype, nsteps, time_block):
+    """Steps per launch, in launch order, of an advect of nsteps steps with
+    this time_block (0 = TIME_BLOCK_AUTO): fdb_plan_passes."""
+    nd = len(num_cells)
+    n = C.c_int()
+    check(lib().fdb_plan_passes(nd, u64s(num_cells), dbls(velocity[:nd]), dbls(lengths[:nd]),
+                                float(dt), int(dtype), int(nsteps), int(time_block), None, 0,
+                                C.byref(n)))
+    ks = (C.c_int * max(1, n.value))()
+    check(lib().fdb_plan_passes(nd, u64s(num_cells), dbls(velocity[:nd]), dbls(lengths[:nd]),
+                                float(dt), int(dtype), int(nsteps), int(time_block), ks, n.value,
+                                C.byref(n)))
+    return list(ks[:n.value])

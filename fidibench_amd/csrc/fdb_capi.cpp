@@ -208,7 +208,8 @@ int run_advect(fdb_upwind* h, int nsteps, double dt, int* launches) {
   int total = 0;
   int left = nsteps;
   while (left > 0) {
-    const int k = std::min(left, h->time_block);
+    const int k = h->time_block == 0 ? fdb::plan_pass(h->dtype, g, left, fdb::kMaxTimeBlock)
+                                     : std::min(left, h->time_block);
     int l = 0;
     FDB_HIP(fdb::launch_advect_pass(h->dtype, g, h->buf, &h->cur, k, h->stream, &l,
                                     h->steps_done, h->seed));
@@ -548,8 +549,8 @@ int fdb_upwind_set_seed(fdb_upwind_t* h, uint64_t seed) {
 
 int fdb_upwind_set_time_block(fdb_upwind_t* h, int k) {
   if (!h) return fail(FDB_ERR_INVALID, "set_time_block: null handle");
-  if (k < 1 || k > fdb::kMaxTimeBlock)
-    return fail(FDB_ERR_INVALID, "set_time_block: k out of range");
+  if (k < 0 || k > fdb::kMaxTimeBlock)
+    return fail(FDB_ERR_INVALID, "set_time_block: k out of range (0..8)");
   h->time_block = k;
   return FDB_OK;
 }
@@ -565,15 +566,16 @@ int fdb_advect_dev_seeded(int ndims, const uint64_t* num_cells, const double* ve
     return fail(FDB_ERR_INVALID, "advect_dev: invalid geometry");
   if (!valid_dtype(dtype)) return fail(FDB_ERR_INVALID, "advect_dev: unknown dtype");
   if (!a || !b || a == b) return fail(FDB_ERR_INVALID, "advect_dev: need two distinct buffers");
-  if (time_block < 1 || time_block > fdb::kMaxTimeBlock)
-    return fail(FDB_ERR_INVALID, "advect_dev: time_block out of range");
+  if (time_block < 0 || time_block > fdb::kMaxTimeBlock)
+    return fail(FDB_ERR_INVALID, "advect_dev: time_block out of range (0..8)");
   if (int rc = check_tuning()) return rc;
   void* const bufs[2] = {a, b};
   int cur = 0;
   int left = nsteps;
   uint64_t stepno = step0;  // keys the stochastic rounding of each step
   while (left > 0) {
-    const int k = std::min(left, time_block);
+    const int k = time_block == 0 ? fdb::plan_pass(dtype, g, left, fdb::kMaxTimeBlock)
+                                  : std::min(left, time_block);
     FDB_HIP(fdb::launch_advect_pass(dtype, g, bufs, &cur, k, (hipStream_t)stream, nullptr,
                                     stepno, seed));
     stepno += (uint64_t)k;
@@ -588,6 +590,39 @@ int fdb_advect_dev(int ndims, const uint64_t* num_cells, const double* velocity,
                    int nsteps, int time_block, void* stream, int* result_in_b) {
   return fdb_advect_dev_seeded(ndims, num_cells, velocity, lengths, delta_time, dtype, a, b,
                                nsteps, time_block, 0, 0, stream, result_in_b);
+}
+
+int fdb_plan_passes(int ndims, const uint64_t* num_cells, const double* velocity,
+                    const double* lengths, double delta_time, int dtype, int nsteps,
+                    int time_block, int* ks, int max_ks, int* count) {
+  fdb::Grid3 g;
+  if (!fdb::make_grid(ndims, num_cells, velocity, lengths, delta_time, &g))
+    return fail(FDB_ERR_INVALID, "plan_passes: invalid geometry");
+  if (!valid_dtype(dtype)) return fail(FDB_ERR_INVALID, "plan_passes: unknown dtype");
+  if (time_block < 0 || time_block > fdb::kMaxTimeBlock)
+    return fail(FDB_ERR_INVALID, "plan_passes: time_block out of range (0..8)");
+  if (nsteps < 0 || !count || (max_ks > 0 && !ks))
+    return fail(FDB_ERR_INVALID, "plan_passes: invalid arguments");
+  if (int rc = check_tuning()) return rc;
+  int n = 0;
+  for (int left = nsteps; left > 0;) {
+    int k = time_block == 0 ? fdb::plan_pass(dtype, g, left, fdb::kMaxTimeBlock)
+                            : std::min(left, time_block);
+    left -= k;
+    while (k > 0) {  // the launches of one pass: deepest supported depth first
+      int K = 1;
+      for (int c : {8, 7, 6, 4, 3, 2})
+        if (c <= k && fdb::fused_pass_supported(dtype, g, c)) {
+          K = c;
+          break;
+        }
+      if (n < max_ks) ks[n] = K;
+      ++n;
+      k -= K;
+    }
+  }
+  *count = n;
+  return FDB_OK;
 }
 
 int fdb_adjoint_step_dev(int ndims, const uint64_t* num_cells, const double* velocity,

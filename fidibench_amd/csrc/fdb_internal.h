@@ -82,8 +82,14 @@ hipError_t launch_fused_pass(int dtype, const Grid3& g, const void* src, void* d
                              uint64_t seed = 0);
 
 // Whether launch_fused_pass would run k fused steps of this storage type
-// (FDB_F64, FDB_F32, FDB_BF16) and shape in one launch.
+// (FDB_F64, FDB_F32, FDB_BF16, FDB_BF16_SR) and shape in one launch.
 bool fused_pass_supported(int dtype, const Grid3& g, int k);
+
+// Steps of the next launch when `left` (>= 1) steps remain and at most
+// `cap` may be fused: the first pass of the least-time split of `left`
+// into one-step launches and supported fused passes (1 = a one-step
+// launch).  Every split gives the same results.
+int plan_pass(int dtype, const Grid3& g, int left, int cap);
 
 // The storage layout of a dtype (FDB_BF16_SR is stored as FDB_BF16).
 inline int storage_dtype(int dtype) { return dtype == 3 ? 2 : dtype; }

@@ -1757,6 +1757,59 @@ bool fused_pass_supported(int dtype, const Grid3& g, int k) {
          g.n[2] * (int64_t)bytes <= INT32_MAX;
 }
 
+namespace {
+// Device time (us) of one launch at 512^3 on MI355X: index 1 the one-step
+// kernel, 2..8 the fused pass of that many steps (0: no such pass).
+// Rows: FP64, FP32, bf16 round to nearest.
+constexpr float kPassUs[3][kMaxTimeBlock + 1] = {
+    {0, 377, 576, 583, 553, 0, 563, 638, 734},
+    {0, 192, 379, 403, 381, 0, 400, 450, 407},
+    {0, 101, 232, 229, 246, 0, 396, 352, 448}};
+constexpr int kPlanTail = 48;  // steps split exactly; longer runs first take the best depth
+}  // namespace
+
+int plan_pass(int dtype, const Grid3& g, int left, int cap) {
+  cap = std::min(std::min(cap, left), kMaxTimeBlock);
+  if (cap <= 1) return 1;
+  bool ok[kMaxTimeBlock + 1] = {};
+  ok[1] = true;
+  for (int k = 2; k <= cap; ++k) ok[k] = fused_pass_supported(dtype, g, k);
+  if (dtype == FDB_BF16_SR) {
+    // Stochastic-rounding passes are bound by the hash arithmetic: about
+    // the one-step rate at 512^3, faster on larger grids.  The deepest
+    // pass that fits.
+    for (int k = cap; k >= 2; --k)
+      if (ok[k]) return k;
+    return 1;
+  }
+  const float* us = kPassUs[dtype == FDB_F64 ? 0 : (dtype == FDB_F32 ? 1 : 2)];
+  for (int k = 2; k <= cap; ++k) ok[k] = ok[k] && us[k] > 0;
+  if (left > kPlanTail) {  // the least time per step
+    int best = 1;
+    for (int k = 2; k <= cap; ++k)
+      if (ok[k] && us[k] * best < us[best] * k) best = k;
+    return best;
+  }
+  // Least total time over the splits of `left` (deeper first on ties).
+  float t[kPlanTail + 1];
+  int first[kPlanTail + 1];
+  t[0] = 0.0f;
+  first[0] = 0;
+  for (int n = 1; n <= left; ++n) {
+    t[n] = -1.0f;
+    first[n] = 1;
+    for (int k = std::min(n, cap); k >= 1; --k) {
+      if (!ok[k]) continue;
+      const float c = t[n - k] + us[k];
+      if (t[n] < 0.0f || c < t[n]) {
+        t[n] = c;
+        first[n] = k;
+      }
+    }
+  }
+  return first[left];
+}
+
 const char* tuning_error() {
   const Tuning& t = tuning();
   return t.error.empty() ? nullptr : t.error.c_str();

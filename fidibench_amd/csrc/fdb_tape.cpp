@@ -185,19 +185,11 @@ int step_into(fdb_tape* t, void* dst, hipStream_t s) {
 // A scratch buffer that is not the working state.
 void* free_scratch(fdb_tape* t) { return t->w == t->scratch[0] ? t->scratch[1] : t->scratch[0]; }
 
-// Steps of the next fused pass over `left` remaining steps (1 = a one-step
-// launch).  Each storage type has a fastest depth per step at 512^3
-// (FP64 and bf16: 6, FP32 and bf16 with stochastic rounding: 8); passes use
-// it while at least two such passes remain, and a remainder of at most 8
-// steps goes into the deepest single pass that fits.
+// Steps of the next launch over `left` remaining steps (1 = a one-step
+// launch): the least-time split of the library's pass planner, at most
+// `cap` steps per pass.
 int pass_steps(int dtype, const fdb::Grid3& g, int left, int cap) {
-  const int pref = (dtype == FDB_F32 || dtype == FDB_BF16_SR) ? 8 : 6;
-  if (left >= 2 * pref || left > fdb::kMaxTimeBlock) {
-    if (pref <= cap && fdb::fused_pass_supported(dtype, g, pref)) return pref;
-  }
-  for (int K : {8, 7, 6, 4, 3, 2})
-    if (K <= left && K <= cap && fdb::fused_pass_supported(dtype, g, K)) return K;
-  return 1;
+  return fdb::plan_pass(dtype, g, left, cap);
 }
 
 // Advance the working state by a steps; the last one lands in `last_dst`

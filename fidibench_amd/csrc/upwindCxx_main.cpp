@@ -29,7 +29,7 @@ int main(int argc, char** argv) {
   args.set("-vtk", false, "Write output to VTK file");
   args.set("-std", false, "Print out spread of solution");
   args.set("-precision", "f64", "Field storage: f64 (bit-exact), f32, bf16 or bf16sr (stochastic rounding)");
-  args.set("-timeBlock", 1, "Time steps fused per pass over the field");
+  args.set("-timeBlock", 1, "Time steps fused per pass over the field (0: automatic)");
   args.setFooter("\nfidibench-amd: upwind advection on the GPU\n");
 
   const bool success = args.parse(argc, argv);

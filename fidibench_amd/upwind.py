@@ -122,7 +122,15 @@ class Upwind:
         L.check(L.lib().fdb_upwind_set_seed(self._h, int(seed)))
 
     def set_time_block(self, k: int) -> None:
+        """Steps fused per pass (1..8), or 0 (TIME_BLOCK_AUTO) for the
+        library's least-time split of each advect; same results for all."""
         L.check(L.lib().fdb_upwind_set_time_block(self._h, int(k)))
+
+    def plan_passes(self, numTimeSteps: int, deltaTime: float, time_block: int) -> list[int]:
+        """Steps per launch of advect(numTimeSteps, deltaTime) with this
+        time_block (no device work)."""
+        return L.plan_passes(self.numCells, self.velocity, self.lengths, deltaTime, self.dtype,
+                             numTimeSteps, time_block)
 
     def advect_timed(self, numTimeSteps: int, deltaTime: float) -> tuple[float, int]:
         """advect(); returns (device milliseconds of its kernels, kernel launches)."""

@@ -115,6 +115,8 @@ class Upwind {
     detail::check(fdb_upwind_mass(h_, &r), "mass");
     return r;
   }
+  // Steps fused per pass: 1..8, or FDB_TIME_BLOCK_AUTO (0) for the
+  // library's least-time split of each advect; same results for every k.
   void setTimeBlock(int k) { detail::check(fdb_upwind_set_time_block(h_, k), "setTimeBlock"); }
   fdb_upwind_t* handle() const { return h_; }
 

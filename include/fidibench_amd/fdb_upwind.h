@@ -131,9 +131,13 @@ int fdb_upwind_init_field(fdb_upwind_t* h, int kind, const double* params, int n
 /* Sum of the field by an FP64 compensated (Neumaier + TwoSum) reduction
  * on the device: the mass used for drift reports. */
 int fdb_upwind_mass(fdb_upwind_t* h, double* out);
-/* Steps fused per HBM pass in advect (temporal blocking, 1..8; FP64, FP32
- * and FDB_BF16 -- FDB_BF16_SR always steps one by one); 1 = one kernel
- * launch per step.  Results are identical for every k. */
+/* Steps fused per HBM pass in advect (temporal blocking): 1..8 = passes of
+ * at most k steps (1 = one kernel launch per step, the default);
+ * FDB_TIME_BLOCK_AUTO (0) = the library's least-time split of each advect
+ * into one-step launches and fused passes (fdb_plan_passes lists it).
+ * Results are identical for every k.  Extension (no counterpart in the
+ * reference). */
+#define FDB_TIME_BLOCK_AUTO 0
 int fdb_upwind_set_time_block(fdb_upwind_t* h, int k);
 /* Seed of the stochastic rounding of an FDB_BF16_SR handle (default 0);
  * also restarts the handle's step count, which indexes the random bits. */
@@ -148,10 +152,21 @@ int fdb_upwind_advect_timed(fdb_upwind_t* h, int num_time_steps, double delta_ti
 /* nsteps of advect on a caller-owned field: steps alternate between the
  * buffers a and b (same size, both device memory), starting from a.
  * *result_in_b is set to 1 if the final state is in b, else 0.  The
- * arguments mean what they mean in fdb_upwind_create/advect. */
+ * arguments mean what they mean in fdb_upwind_create/advect;
+ * time_block as in fdb_upwind_set_time_block (0..8). */
 int fdb_advect_dev(int ndims, const uint64_t* num_cells, const double* velocity,
                    const double* lengths, double delta_time, int dtype, void* a, void* b,
                    int nsteps, int time_block, void* stream, int* result_in_b);
+
+/* The pass depths (steps per launch, in launch order) that an advect of
+ * nsteps steps with this time_block (0..8, 0 = FDB_TIME_BLOCK_AUTO) makes
+ * for this geometry and dtype, as fdb_upwind_advect and fdb_advect_dev
+ * run them: *count gets the number of launches, ks[0 .. min(count,
+ * max_ks)-1] their depths.  Host-only (no device work).  Extension (no
+ * counterpart in the reference). */
+int fdb_plan_passes(int ndims, const uint64_t* num_cells, const double* velocity,
+                    const double* lengths, double delta_time, int dtype, int nsteps,
+                    int time_block, int* ks, int max_ks, int* count);
 
 /* fdb_advect_dev with the stochastic-rounding stream made explicit: for
  * FDB_BF16_SR the random bits of the i-th step of this call are keyed by

@@ -361,3 +361,69 @@ def test_plan_400_steps_few_snapshots(snaps):
     while comb(s + r, s) < T:
         r += 1
     assert fw == r * T - comb(s + r, s + 1), (snaps, r, fw)
+
+
+# ------------------------------------------------------------ pass planner
+
+SN512 = ((512, 512, 512), (0.7, -0.4, 0.2), (1.0, 1.0, 1.0))
+
+
+def _passes(dtype, nsteps, tb, geom=SN512, dt=1e-4):
+    L = _lib()
+    shape, v, lens = geom
+    return L.plan_passes(shape, v, lens, dt, dtype, nsteps, tb)
+
+
+def test_auto_plan_of_twenty_fp64_steps_is_seven_seven_six():
+    """20 FP64 steps at 512^3: 7+7+6 (three passes of near-best depth per
+    step) rather than 8+8+4, whose 4-step pass costs as much as a 6-step
+    one (launch times measured on MI355X)."""
+    L = _lib()
+    assert sorted(_passes(L.F64, 20, 0), reverse=True) == [7, 7, 6]
+
+
+@pytest.mark.parametrize("dtype", ["F64", "F32", "BF16", "BF16_SR"])
+@pytest.mark.parametrize("nsteps", [1, 2, 5, 9, 13, 20, 47, 48, 49, 100, 400])
+def test_auto_plan_covers_every_step_with_fused_depths(dtype, nsteps):
+    L = _lib()
+    ks = _passes(getattr(L, dtype), nsteps, 0)
+    assert sum(ks) == nsteps
+    allowed = {1, 2, 4, 8} if dtype == "BF16_SR" else {1, 2, 3, 4, 6, 7, 8}
+    assert set(ks) <= allowed, ks
+    if nsteps >= 8:  # long runs are mostly deep passes
+        assert sum(k for k in ks if k >= 6) >= nsteps - 8, ks
+
+
+def test_auto_plan_takes_few_launches():
+    """The automatic split of n FP64 steps at 512^3 takes at most as many
+    launches as passes of 6 steps would (a one-step launch, about half a
+    fused pass's time, may close it: 9 steps = 8 + 1)."""
+    L = _lib()
+    for n in range(9, 120):
+        auto = _passes(L.F64, n, 0)
+        assert len(auto) <= -(-n // 6), (n, auto)
+    assert _passes(L.F64, 9, 0) == [8, 1]
+
+
+def test_fixed_time_block_plan_splits_unsupported_depths():
+    """time_block k: passes of min(left, k) steps, each launched as the
+    deepest supported fused depth first (no 5-step kernel: 4 + 1)."""
+    L = _lib()
+    assert _passes(L.F64, 12, 5) == [4, 1, 4, 1, 2]
+    assert _passes(L.F64, 10, 1) == [1] * 10
+    assert _passes(L.F64, 17, 8) == [8, 8, 1]
+
+
+def test_plan_on_a_grid_without_fused_passes_is_one_step_launches():
+    """A 1-D grid has no fused form: every step is a one-step launch."""
+    L = _lib()
+    geom = ((1000,), (0.5,), (1.0,))
+    assert _passes(L.F64, 9, 0, geom) == [1] * 9
+
+
+@pytest.mark.parametrize("tb", [-1, 9])
+def test_plan_time_block_out_of_range_is_invalid(tb):
+    L = _lib()
+    with pytest.raises(L.FdbError) as e:
+        _passes(L.F64, 10, tb)
+    assert e.value.code == L.ERR_INVALID

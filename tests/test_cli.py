@@ -75,7 +75,7 @@ def test_runs_match_reference_stdout(manifest):
 @pytest.mark.gpu
 @pytest.mark.parametrize("extra", [["-precision", "f32"], ["-precision", "bf16"],
                                    ["-precision", "bf16sr"], ["-timeBlock", "4"],
-                                   ["-timeBlock", "8"]])
+                                   ["-timeBlock", "8"], ["-timeBlock", "0"]])
 def test_extension_options(extra):
     p = run(["-numCells", "32", "-numSteps", "10"] + extra)
     assert p.returncode == 0, p.stderr

@@ -38,6 +38,32 @@ def test_time_blocked_fp64_bit_exact(oracle, shape, signs):
         up.close()
 
 
+@pytest.mark.parametrize("dtype", ["f64", "f32", "bf16", "bf16sr"])
+@pytest.mark.parametrize("steps", [9, 23])
+def test_auto_time_block_runs_the_plan_bit_exact(dtype, steps):
+    """time_block 0 (automatic): advect launches exactly the planner's
+    passes (fdb_plan_passes) and ends on the same bits as one launch per
+    step; the partition changes no value."""
+    from fidibench_amd import Upwind, configs as K
+    shape = (21, 70, 130)
+    v = (0.3, -0.45, 0.2)
+    dt = K.cfl_dt(shape, v)
+    f0 = np.random.default_rng(steps).random(shape)
+    one = Upwind(v, (1, 1, 1), shape, dtype=dtype)
+    one.set_field(f0)
+    one.advect(steps, dt)
+    auto = Upwind(v, (1, 1, 1), shape, dtype=dtype)
+    auto.set_field(f0)
+    auto.set_time_block(0)
+    plan = auto.plan_passes(steps, dt, 0)
+    ms, launches = auto.advect_timed(steps, dt)
+    assert sum(plan) == steps and launches == len(plan), (plan, launches)
+    assert max(plan) > 1  # fused passes were planned
+    assert np.array_equal(bits(auto.get_field()), bits(one.get_field())), (dtype, steps, plan)
+    one.close()
+    auto.close()
+
+
 @pytest.mark.parametrize("shape", SHAPES[:2])
 @pytest.mark.parametrize("signs", list(itertools.product((1, -1), repeat=3)))
 def test_time_blocked_bf16_bit_exact(oracle, shape, signs):
