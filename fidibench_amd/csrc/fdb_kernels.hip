@@ -935,7 +935,14 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
 #pragma unroll
     for (int r = 0; r < R; ++r) nx[r] = x[r];  // defined where a prefetch is skipped
   }
-  constexpr bool kCondPrefetch = PIPE && K > 6;
+  // Two planes per trip of the plane loop (the registers of prev[l] and of
+  // level l alternate between planes instead of being copied at the end of
+  // each plane) where that fits the registers: K <= 6, and the FP64
+  // shapes sized for it, 56 row-levels per wave (8 rows at K = 7, 7 at
+  // K = 8).  (FP64 with 9 rows at K = 7, 8 would spill; the 5-row K = 8
+  // shape keeps one plane per trip within 128 VGPRs.)
+  constexpr bool kTwoPlanes = !PROD && (K <= 6 || (sizeof(M) == 8 && R * K == 56));
+  constexpr bool kCondPrefetch = PIPE && !kTwoPlanes;
   constexpr bool kWaitBeforeStores = (MODE == kTbPlain || MODE == kTbSR) && !kCondPrefetch;
   // PROD: fv[l] holds the state of level l at the current plane; after
   // level l it is reloaded with the next plane's (always a valid plane, so
@@ -964,9 +971,7 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
     }
   }
   // One plane, all K levels; each level in place: row r from level l to
-  // l+1 with axis-1 neighbour n1.  For K <= 6 (forward modes) the plane
-  // loop runs two planes per trip so that the registers of prev[l] and of
-  // level l alternate between planes instead of being copied.
+  // l+1 with axis-1 neighbour n1 (two planes per trip with kTwoPlanes).
   int lvl = 0;
   auto plane_step = [&](const int64_t m) __attribute__((always_inline)) {
     int64_t pn = pw + step;
@@ -1180,14 +1185,15 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
     lds_barrier();
   }
   int64_t m = 0;
-  if constexpr (K <= 6 && !PROD) {
+  if constexpr (kTwoPlanes) {
     for (; m + 1 < cnt; m += 2) {
       plane_step(m);
       plane_step(m + 1);
     }
     if (m < cnt) plane_step(m);
-  } else {  // K = 7, 8 and the adjoint: two planes per trip would spill
-            // (FP64, adjoint) or lose a wave per SIMD (FP32, BF16)
+  } else {  // two planes per trip would spill (FP64 9 rows at K = 7, 8;
+            // the adjoint) or lose a wave per SIMD (FP32, BF16 at K = 7, 8;
+            // FP64 5 rows at K = 8)
     for (; m < cnt; ++m) plane_step(m);
   }
   if constexpr (PROD) block_reduce_vacc<64 * WY>(acc, adj.partials);
@@ -1286,6 +1292,8 @@ constexpr int kRowsShapes[][2] = {{8, 2}, {4, 2}, {4, 4}, {8, 1}, {2, 4}, {2, 8}
 // (R, K, WY) of the temporally blocked forward (FDB_TB below).
 constexpr int kTbShapes[][3] = {{9, 2, 4}, {9, 3, 4}, {9, 4, 4}, {9, 6, 8},
                                 {9, 7, 8}, {9, 8, 8}, {5, 8, 8}};
+// FP64-only shapes: two planes per trip fit in 256 VGPRs.
+constexpr int kTbShapes64[][3] = {{8, 7, 8}, {7, 8, 8}};
 
 // Default launch shape of the rows kernel.  bytes: storage of the stepped
 // field (lam for the adjoint); fbytes: the adjoint's f_t storage (0: none).
@@ -1317,7 +1325,13 @@ RowsCfg rows_default(int bytes, bool adj, int fbytes) {
   return c;
 }
 
-TbCfg tb_default(int K) { return K >= 6 ? TbCfg{9, 8, 0} : TbCfg{9, 4, 0}; }
+// FP64 at K = 7: 8 rows x 8 waves, two planes per plane-loop trip (512^3:
+// 1516 vs 1431 G cell-updates/s for 9 rows, one plane per trip; at K = 8
+// the 7-row two-plane shape loses, 1395 vs 1480 G).
+TbCfg tb_default(int bytes, int K) {
+  if (bytes == 8 && K == 7) return TbCfg{8, 8, 0};
+  return K >= 6 ? TbCfg{9, 8, 0} : TbCfg{9, 4, 0};
+}
 
 // Launch shapes, read once per process from the environment (a
 // function-local static, so concurrent first calls are safe).  Tuning
@@ -1341,10 +1355,13 @@ bool rows_shape_ok(const RowsCfg& c) {
   return false;
 }
 
-bool tb_shape_ok(const TbCfg& c, int K) {
+bool tb_shape_ok(const TbCfg& c, int K, int bytes) {
   if (c.blocks < 0) return false;
   for (const auto& s : kTbShapes)
     if (s[0] == c.R && s[1] == K && s[2] == c.WY) return true;
+  if (bytes == 8)
+    for (const auto& s : kTbShapes64)
+      if (s[0] == c.R && s[1] == K && s[2] == c.WY) return true;
   return false;
 }
 
@@ -1406,11 +1423,11 @@ Tuning load_tuning() {
       t.rows[1][b][fb] = a;
     }
     for (int K = 2; K <= kMaxTimeBlock; ++K) {
-      TbCfg c = tb_default(K);
+      TbCfg c = tb_default(b, K);
       snprintf(name, sizeof(name), "FDB_TB_CFG_%d_%d", b, K);
       if (const char* s = getenv(name)) {
         int v[3] = {0, 0, 0};
-        if (parse_ints(s, v, 3, 3) == 3 && tb_shape_ok(TbCfg{v[0], v[1], v[2]}, K))
+        if (parse_ints(s, v, 3, 3) == 3 && tb_shape_ok(TbCfg{v[0], v[1], v[2]}, K, b))
           c = TbCfg{v[0], v[1], v[2]};
         else
           bad(name, s);
@@ -1604,6 +1621,8 @@ hipError_t tb_dispatch(const Grid3& g, const TS* in, TS* out, int K, hipStream_t
   FDB_TB(9, 8, 8, kTbPlain)
   FDB_TB(5, 8, 8, kTbPlain)
   if constexpr (sizeof(TS) == 8) {
+    FDB_TB(8, 7, 8, kTbPlain)
+    FDB_TB(7, 8, 8, kTbPlain)
     FDB_TB(9, 4, 4, kTbStoreAll)
     FDB_TB(9, 8, 8, kTbStoreAll)
   }
@@ -1753,7 +1772,7 @@ bool fused_pass_supported(int dtype, const Grid3& g, int k) {
   const bool sr = dtype == FDB_BF16_SR;
   if (sr && !tb_sr_ok(g, k)) return false;
   const TbCfg c = sr ? tb_sr_cfg(k) : tb_cfg(bytes, k);
-  return (sr || tb_shape_ok(c, k)) && g.act[0] && g.act[1] && g.act[2] && c.WY * c.R > k &&
+  return (sr || tb_shape_ok(c, k, bytes)) && g.act[0] && g.act[1] && g.act[2] && c.WY * c.R > k &&
          g.n[2] * (int64_t)bytes <= INT32_MAX;
 }
 
@@ -1762,7 +1781,7 @@ namespace {
 // kernel, 2..8 the fused pass of that many steps (0: no such pass).
 // Rows: FP64, FP32, bf16 round to nearest.
 constexpr float kPassUs[3][kMaxTimeBlock + 1] = {
-    {0, 377, 576, 583, 553, 0, 563, 638, 734},
+    {0, 382, 572, 579, 545, 0, 558, 602, 716},
     {0, 192, 379, 403, 381, 0, 400, 450, 407},
     {0, 101, 232, 229, 246, 0, 396, 352, 448}};
 constexpr int kPlanTail = 48;  // steps split exactly; longer runs first take the best depth
