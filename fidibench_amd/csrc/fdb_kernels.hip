@@ -62,11 +62,14 @@ template <> struct St<uint16_t> {
 // Stochastic rounding of step results for bf16 storage (FDB_BF16_SR): 16
 // random bits are added below the kept bits of the binary32 value, then
 // truncated, so x rounds up with probability equal to the discarded
-// fraction (unbiased).  The bits are one half of lowbias32 (C. Wellons'
-// 32-bit integer hash) of the index of the cell's pair (flat index / 2)
-// mixed with a per-step key made on the host from the seed and the step's
-// index: the low half for an even cell, the high half for an odd one (one
-// hash per two cells).  Inf stays Inf, NaN stays quiet.
+// fraction (unbiased).  The bits are one half of a pair word: lowbias32
+// (C. Wellons' 32-bit integer hash) of the index of the cell's pair (flat
+// index / 2), xor a per-step key made on the host from the seed and the
+// step's index, through one multiply-xorshift round (sr_round): the low
+// half for an even cell, the high half for an odd one (one word per two
+// cells).  The pair's lowbias32 does not depend on the step, so a fused
+// pass hashes each pair once per plane and pays one multiply per pair and
+// step.  Inf stays Inf, NaN stays quiet.
 __host__ __device__ inline uint32_t lowbias32(uint32_t x) {
   x ^= x >> 16;
   x *= 0x7feb352du;
@@ -76,8 +79,22 @@ __host__ __device__ inline uint32_t lowbias32(uint32_t x) {
   return x;
 }
 
+// x ^= x >> 16, x *= 0x7feb352d, x ^= x >> 16: lowbias32's first round
+// and its closing shift, which carries the product's high half into the
+// low half (the even cell's bits).
+__device__ inline uint32_t sr_round(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  return x ^ (x >> 16);
+}
+
+// lowbias32 of a pair index folded to 32 bits: the step-independent part.
+__device__ inline uint32_t sr_pair_word(uint64_t pair) {
+  return lowbias32((uint32_t)(pair ^ (pair >> 32)));
+}
+
 __device__ inline uint32_t sr_pair_hash(uint64_t pair, uint32_t key) {
-  return lowbias32((uint32_t)(pair ^ (pair >> 32)) ^ key);
+  return sr_round(sr_pair_word(pair) ^ key);
 }
 
 // x rounded with 16 random bits r16: (bits(x) + r16) >> 16 for finite x
@@ -807,8 +824,8 @@ struct TbArgs {
 //                per thread and reduced per block into aux.partials[block*6].
 //   kTbSR        K forward steps in bf16 storage, each level rounded
 //                stochastically with the bits a one-step launch of that step
-//                draws (lowbias32 of the cell's pair index and the step's
-//                key, low half for an even cell, high half for an odd one),
+//                draws (sr_pair_hash of the cell's pair and the step's key,
+//                low half for an even cell, high half for an odd one),
 //                so the result is the one-step launches' bit for bit.  The
 //                two lanes of a cell pair (even row length, even K: lane
 //                parity = column parity) split the hashing: each hashes half
@@ -982,7 +999,7 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
         // (row offsets are uniform: the lane's row is picked per plane)
         const int64_t ro = odd ? roff[2 * j + 1 < R ? 2 * j + 1 : R - 1] : roff[2 * j];
         const uint64_t pair = ((uint64_t)(pw * plane + ro) + col) >> 1;
-        sword[j] = (uint32_t)(pair ^ (pair >> 32));
+        sword[j] = sr_pair_word(pair);  // once per plane, for all K levels
       }
     }
     if (!kCondPrefetch || m + 1 < cnt) {
@@ -1137,7 +1154,7 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
           // conversion writes the high half of a register whose low half is
           // kept, so it is given a dead bf16 value (low half zero).
           if (r % 2 == 0) {
-            own = lowbias32(sword[r / 2] ^ a.sr_key[l]);
+            own = sr_round(sword[r / 2] ^ a.sr_key[l]);
             par = __builtin_amdgcn_mov_dpp(own, 0xb1, 0xf, 0xf, true);
           }
           const uint32_t rbits = __builtin_amdgcn_perm(own, par, ssel[r & 1]);
@@ -1794,9 +1811,9 @@ int plan_pass(int dtype, const Grid3& g, int left, int cap) {
   ok[1] = true;
   for (int k = 2; k <= cap; ++k) ok[k] = fused_pass_supported(dtype, g, k);
   if (dtype == FDB_BF16_SR) {
-    // Stochastic-rounding passes are bound by the hash arithmetic: about
-    // the one-step rate at 512^3, faster on larger grids.  The deepest
-    // pass that fits.
+    // Stochastic-rounding passes are bound by the hash arithmetic, and the
+    // deepest is the fastest per step (512^3: 110 us at K = 8, 115 at 4,
+    // 153 at 2, 120 one step per launch).  The deepest pass that fits.
     for (int k = cap; k >= 2; --k)
       if (ok[k]) return k;
     return 1;

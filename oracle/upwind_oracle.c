@@ -214,9 +214,10 @@ int orc_advect_bf16(int nd, const uint64_t* n, const double* v, const double* le
 /* bfloat16 storage with stochastic rounding of every step's result: 16
  * random bits added below the kept bits of the binary32 value, then
  * truncated.  The bits come from lowbias32 (C. Wellons' integer hash) of
- * the folded index of the cell's pair (cell / 2) xor a per-step key hashed
- * from (seed, step): the low half for an even cell, the high half for an
- * odd one. */
+ * the folded index of the cell's pair (cell / 2), xor a per-step key hashed
+ * from (seed, step), through one multiply-xorshift round (lowbias32's first
+ * round and closing shift): the low half for an even cell, the high half
+ * for an odd one. */
 static uint32_t orc_lowbias32(uint32_t x) {
   x ^= x >> 16;
   x *= 0x7feb352du;
@@ -231,9 +232,15 @@ uint32_t orc_sr_step_key(uint64_t seed, uint64_t step) {
   return orc_lowbias32(k1 ^ (uint32_t)seed ^ ((uint32_t)(seed >> 32) * 0x85EBCA6Bu));
 }
 
+static uint32_t orc_sr_round(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  return x ^ (x >> 16);
+}
+
 uint32_t orc_sr_bits(uint64_t cell, uint32_t key) {
   const uint64_t pr = cell >> 1;
-  const uint32_t h = orc_lowbias32((uint32_t)(pr ^ (pr >> 32)) ^ key);
+  const uint32_t h = orc_sr_round(orc_lowbias32((uint32_t)(pr ^ (pr >> 32))) ^ key);
   return (cell & 1u) ? (h >> 16) : (h & 0xffffu);
 }
 

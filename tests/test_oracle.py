@@ -192,10 +192,16 @@ def _lowbias32(x):
     return x
 
 
+def _sr_round(x):
+    x ^= x >> 16
+    x = (x * 0x7FEB352D) & 0xFFFFFFFF
+    return x ^ (x >> 16)
+
+
 def test_bf16_stochastic_rounding_oracle(oracle):
     """Each result is one of the two bf16 neighbours of x; bf16-exact values
     are kept; the mean over many cells is x (unbiased); the random bits are
-    lowbias32 as published (Python restatement)."""
+    lowbias32 as published and its first round (Python restatement)."""
     L = oracle.lib()
     for seed, step in [(0, 0), (7, 3), (2 ** 40 + 5, 2 ** 33 + 1)]:
         k1 = _lowbias32((step + 0x9E3779B9 * (step >> 32)) & 0xFFFFFFFF)
@@ -218,11 +224,43 @@ def test_bf16_stochastic_rounding_oracle(oracle):
     # Bit layout: one hash per cell pair, low half for the even cell.
     for cell in (0, 1, 2, 3, 12345, 12346, 2 ** 33 + 7, 2 ** 40 + 2):
         pr = cell >> 1
-        h = _lowbias32(((pr ^ (pr >> 32)) & 0xFFFFFFFF) ^ key)
+        h = _sr_round(_lowbias32((pr ^ (pr >> 32)) & 0xFFFFFFFF) ^ key)
         r = (h >> 16) if cell & 1 else (h & 0xFFFF)
         assert L.orc_sr_bits(cell, key) == r
         b = int(np.float32(1.3).view(np.uint32))
         assert L.orc_bf16_sr(1.3, cell, key) == ((b + r) >> 16) & 0xFFFF
+
+
+def _sr_words(L, cells, steps, seed=7):
+    return np.array([[L.orc_sr_bits(int(c), L.orc_sr_step_key(seed, t)) for c in cells]
+                     for t in steps], dtype=np.float64) / 65536.0
+
+
+def test_bf16_sr_bits_are_uniform_and_uncorrelated(oracle):
+    """The 16-bit rounding fractions over 256 cells x 64 steps: each of 16
+    bins holds its share (chi-square, 15 degrees of freedom), and neither
+    neighbouring cells (even/odd of one pair, and adjacent pairs) nor
+    consecutive steps of one cell, nor any two steps over the whole set of
+    cells, are correlated beyond sampling noise."""
+    L = oracle.lib()
+    cells = np.arange(1000, 1256)
+    u = _sr_words(L, cells, range(64))
+    counts = np.histogram(u, bins=16, range=(0, 1))[0]
+    exp = u.size / 16
+    chi2 = ((counts - exp) ** 2 / exp).sum()
+    assert chi2 < 40, chi2  # p < 1e-3 at 15 dof
+    assert abs(u.mean() - 0.5) < 0.01
+    def corr(a, b):
+        return float(np.corrcoef(a.ravel(), b.ravel())[0, 1])
+    lim = 4.0 / np.sqrt(u.size / 2)
+    assert abs(corr(u[:, 0::2], u[:, 1::2])) < lim      # the two cells of a pair
+    assert abs(corr(u[:, 1:-1:2], u[:, 2::2])) < lim    # adjacent pairs
+    assert abs(corr(u[:-1], u[1:])) < lim               # consecutive steps
+    # any two steps: the per-step key is xored into every cell's pair word,
+    # so a structured key difference would correlate whole steps
+    c = np.corrcoef(u)
+    off = c[~np.eye(len(c), dtype=bool)]
+    assert np.abs(off).max() < 4.0 / np.sqrt(len(cells)) * 1.6, np.abs(off).max()
 
 
 def test_bf16_sr_advect_oracle(oracle):
