@@ -88,6 +88,14 @@ __device__ inline uint32_t sr_round(uint32_t x) {
   return x ^ (x >> 16);
 }
 
+// sr_round(w ^ k) with its first, xor-linear step done on each operand
+// beforehand: sr_round_pre(w ^ (w >> 16) ^ k ^ (k >> 16)) == sr_round(w ^ k).
+__host__ __device__ inline uint32_t sr_fold(uint32_t x) { return x ^ (x >> 16); }
+__device__ inline uint32_t sr_round_pre(uint32_t x) {
+  x *= 0x7feb352du;
+  return x ^ (x >> 16);
+}
+
 // lowbias32 of a pair index folded to 32 bits: the step-independent part.
 __device__ inline uint32_t sr_pair_word(uint64_t pair) {
   return lowbias32((uint32_t)(pair ^ (pair >> 32)));
@@ -806,7 +814,7 @@ struct TbArgs {
   int u0;
   int64_t chunk;               // output planes per tile along axis 0
   int64_t tiles2, tiles1, ntiles;
-  uint32_t sr_key[kMaxTimeBlock];  // kTbSR: stochastic-rounding key of level l's step
+  uint32_t sr_key[kMaxTimeBlock];  // kTbSR: sr_fold(key) of level l's step
 };
 
 // Modes of the temporally blocked kernel.
@@ -920,7 +928,7 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
     row_out[r] = (LO1 ? br >= K : br < WY * R - K) && rl >= 0 && rl < a.n1;
   }
   // SR: this lane hashes rows 2j (even lane) or 2j+1 (odd lane) of its
-  // column's pairs; sword[j] is that cell's pair word for the current plane.
+  // column's pairs; sword[j] is that cell's pair word (sr_fold) for the current plane.
   constexpr int J = (R + 1) / 2;
   const bool odd = (lane & 1) != 0;
   uint32_t sword[SR ? J : 1];
@@ -999,7 +1007,7 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
         // (row offsets are uniform: the lane's row is picked per plane)
         const int64_t ro = odd ? roff[2 * j + 1 < R ? 2 * j + 1 : R - 1] : roff[2 * j];
         const uint64_t pair = ((uint64_t)(pw * plane + ro) + col) >> 1;
-        sword[j] = sr_pair_word(pair);  // once per plane, for all K levels
+        sword[j] = sr_fold(sr_pair_word(pair));  // once per plane, for all K levels
       }
     }
     if (!kCondPrefetch || m + 1 < cnt) {
@@ -1154,7 +1162,7 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
           // conversion writes the high half of a register whose low half is
           // kept, so it is given a dead bf16 value (low half zero).
           if (r % 2 == 0) {
-            own = sr_round(sword[r / 2] ^ a.sr_key[l]);
+            own = sr_round_pre(sword[r / 2] ^ a.sr_key[l]);
             par = __builtin_amdgcn_mov_dpp(own, 0xb1, 0xf, 0xf, true);
           }
           const uint32_t rbits = __builtin_amdgcn_perm(own, par, ssel[r & 1]);
@@ -1607,7 +1615,7 @@ hipError_t tb_dispatch(const Grid3& g, const TS* in, TS* out, int K, hipStream_t
   const bool all = levels != nullptr;
   if (all && sizeof(TS) != 8) return hipErrorNotSupported;
   for (int l = 0; all && l + 1 < K; ++l) aux.lv[l] = levels[l];
-  for (int l = 0; sr && l < K; ++l) a.sr_key[l] = sr_keys[l];
+  for (int l = 0; sr && l < K; ++l) a.sr_key[l] = sr_fold(sr_keys[l]);
   const int mode = all ? kTbStoreAll : (sr ? kTbSR : kTbPlain);
   bool launched = false;
 #define FDB_TB_DIR(RR, KK, W, L1, L2, MD)                                                   \
