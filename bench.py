@@ -367,7 +367,8 @@ def adjoint_step_rates(w, reps=10):
 def laplacian(iters=10, with_cpu=True):
     """The stencil filter of cxx/Filter.cpp (one process) on laplacian.cxx's
     workloads: the 5-point 2-D Laplacian at its default 8000^2 and the
-    7-point 3-D one at 512^3, `iters` x (applyFilter; copyOutToIn), input
+    7-point 3-D one at 512^3, and their 4th-order (radius 2) forms, `iters`
+    x (applyFilter; copyOutToIn), input
     prod sin(2 pi x) made on the GPU.  16 B per cell-update compulsory
     (read input, write output); the oracle port (bit-identical result,
     checked) is timed on the host for one iteration as the CPU figure."""
@@ -375,13 +376,17 @@ def laplacian(iters=10, with_cpu=True):
     import numpy as np
     from fidibench_amd import Filter
     out = {}
-    for name, nd, n in (("lap2d_8000", 2, 8000), ("lap3d_512", 3, 512)):
-        st = {tuple([0] * nd): -2.0 * nd}
+    for name, nd, n, order in (("lap2d_8000", 2, 8000, 2), ("lap3d_512", 3, 512, 2),
+                               ("lap2d4_8000", 2, 8000, 4), ("lap3d4_512", 3, 512, 4)):
+        # 2nd order: 1, -2, 1 per axis; 4th order: -1/12, 4/3, -5/2, 4/3, -1/12
+        taps = {1: 1.0} if order == 2 else {1: 16.0 / 12.0, 2: -1.0 / 12.0}
+        st = {tuple([0] * nd): (-2.0 if order == 2 else -30.0 / 12.0) * nd}
         for a in range(nd):
-            for s in (1, -1):
-                o = [0] * nd
-                o[a] = s
-                st[tuple(o)] = 1.0
+            for k, wk in taps.items():
+                for s in (k, -k):
+                    o = [0] * nd
+                    o[a] = s
+                    st[tuple(o)] = wk
         dims = (n,) * nd
         flt = Filter(dims, [0.0] * nd, [1.0] * nd, st)
         flt.set_in_sin()
@@ -399,8 +404,10 @@ def laplacian(iters=10, with_cpu=True):
                               "unit": "GB/s", "frac": achieved * 1e9 / HBM_PEAK,
                               "traffic": pmc_traffic(name,
                                                      "filter_march_kernel_laplacian_stats.md"),
-                              "kernel": ("filter_march_kernel (rows, 2-D)" if nd == 2 else
-                                         "filter_march_kernel (planes, 3-D)"),
+                              "kernel": {"star": "filter_star_kernel (3-D star stencil)",
+                                         "rows": "filter_march_kernel (rows, 2-D)"}.get(
+                                             flt.kernel(),
+                                             f"filter_march_kernel ({flt.kernel()}, 3-D)"),
                               "avg_launch_ms": launch_ms},
                  "checksum_output": flt.computeCheckSum("output")}
         if name == "lap2d_8000" and with_cpu:
@@ -460,7 +467,8 @@ def large_adjoint(shape=(1024, 1024, 768), steps=400):
         free = torch.cuda.mem_get_info()[0]
         sync()
         gpu_busy()
-        g = misfit_gradient(w, target, L.BF16_SR, L.F32, int(free * 0.8), method=method)
+        g = misfit_gradient(w, target, L.BF16_SR, L.F32, int(free * 0.8), method=method,
+                            reps=3 if method == "shift" else 1)
         lam, gv = g.pop("grad_f0"), g.pop("grad_v")
         g["grad_f0_error_vs_fp64"] = compare(lam, lam64)
         g["grad_v_rel_error_vs_fp64"] = [abs(a - b) / abs(b) for a, b in zip(gv, gv64)]

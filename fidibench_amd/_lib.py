@@ -88,6 +88,7 @@ SIGNATURES = {
     "fdb_filter_apply": (_i, [_vp]),
     "fdb_filter_copy_out_to_in": (_i, [_vp]),
     "fdb_filter_iterate_timed": (_i, [_vp, _i, C.POINTER(C.c_float)]),
+    "fdb_filter_kernel": (_i, [_vp, C.POINTER(C.c_char_p)]),
     "fdb_filter_checksum": (_i, [_vp, _i, _dp]),
     "fdb_filter_get_data": (_i, [_vp, _i, _dp]),
     "fdb_filter_save_vtk": (_i, [_vp, C.c_char_p]),

@@ -211,6 +211,19 @@ int fdb_filter_get_data(fdb_filter_t* h, int which, double* host) {
   return FDB_OK;
 }
 
+int fdb_filter_kernel(const fdb_filter_t* h, const char** name) {
+  if (!h || !name) return ffail(FDB_ERR_INVALID, "filter_kernel: null argument");
+  switch (h->plan.kind) {
+    case fdb::kFilterStar: *name = "star"; break;
+    case fdb::kFilterPlanes: *name = "planes"; break;
+    case fdb::kFilterPlanesMid: *name = "planes_mid"; break;
+    case fdb::kFilterPlanesWide: *name = "planes_wide"; break;
+    case fdb::kFilterRows: *name = "rows"; break;
+    default: *name = "gather"; break;
+  }
+  return FDB_OK;
+}
+
 int fdb_filter_checksum(fdb_filter_t* h, int which, double* out) {
   if (!h || !out || (which != 0 && which != 1))
     return ffail(FDB_ERR_INVALID, "filter_checksum: bad argument");

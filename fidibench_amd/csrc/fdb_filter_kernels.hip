@@ -18,6 +18,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <cstdlib>
 
 #include "fdb_internal.h"
@@ -112,6 +113,11 @@ __device__ inline void lds_barrier() {
 }
 
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+// Marks a loaded value as used here, so the compiler waits for it before
+// this point rather than at a loop header, where the wait it merges for
+// loads issued before the loop would also cover the loop's own stores.
+__device__ inline void settle(double x) { asm volatile("" ::"v"(x)); }
 
 // Every memory access of the plane loop is unconditional, so the compiler's
 // wait counts let the prefetched planes stay in flight: lanes outside the
@@ -229,6 +235,235 @@ __global__ __launch_bounds__(256) void filter_march_kernel(const double* __restr
     if (z + u < z1) step(z + u, v[u]);  // block-uniform
 }
 
+// ---- star kernel: full 3-D star stencils of radius R, the 6R + 1 terms
+// (+-k, 0, 0), (0, +-k, 0), (0, 0, +-k) for k = 1..R and the centre (the
+// 4th-order Laplacian at R = 2; at R = 1 the ring kernel below, whose
+// three planes take 33 KB, measured faster: 334-339 vs 317-319 G
+// cell-updates/s for the 7-point Laplacian at 512^3).  In std::map order their
+// terms are the axis-0 offsets -R..-1, the axis-1 offsets -R..-1, the
+// axis-2 offsets -R..R (the centre among them), the axis-1 offsets 1..R
+// and the axis-0 offsets 1..R; the weights stay runtime values.  The
+// axis-2 terms read only the cell's own column, so each thread keeps its
+// cells' values of planes z-R .. z+R in registers (loaded R planes ahead)
+// and LDS holds only the output plane's halo tile, double-buffered so one
+// barrier per plane suffices (26 KB at R = 2 against the ring kernel's
+// five planes, 45 KB: twice the blocks per CU).  A plane's tile is put
+// together from the threads' own cells (their queue's middle entry) and
+// the halo ring around them, loaded flat across the block, so every value
+// is read from memory once per plane.  Memory accesses in the plane loop
+// are unconditional, as in the ring kernel.
+template <int R>
+struct StarShape {
+  static constexpr int kBX = 64, kBY = 4, kCX = 2, kCY = 2;
+  static constexpr int kTX = kBX * kCX, kTY = kBY * kCY;
+  static constexpr int kLX = kTX + 2 * R, kLY = kTY + 2 * R, kNE = kLX * kLY;
+  static constexpr int kNH = kNE - kTX * kTY;    // halo-ring values per plane
+  static constexpr int kNLH = (kNH + 255) / 256;  // per thread
+  static constexpr size_t kLds = 2 * (kNE + 64) * sizeof(double);
+};
+
+template <int R>
+__global__ __launch_bounds__(256) void filter_star_kernel(const double* __restrict__ in,
+                                                          double* __restrict__ out,
+                                                          FilterGeom g, FilterMarch m) {
+  using S = StarShape<R>;
+  constexpr int BX = S::kBX, BY = S::kBY, CX = S::kCX, CY = S::kCY;
+  constexpr int LX = S::kLX, NE = S::kNE, NH = S::kNH, NLH = S::kNLH, D = 2 * R + 1;
+  static_assert(R == 1 || R == 2, "the plane loop is unrolled for 4 or 6 queue slots");
+  __shared__ double tile[2][NE + 64];  // + a spare area for lanes without a ring value
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int tid = ty * BX + tx;
+  const int64_t b = xcd_contiguous(blockIdx.x, gridDim.x);
+  const int64_t bx = b % m.tiles_x;
+  const int64_t by = (b / m.tiles_x) % m.tiles_y;
+  const int64_t bz = b / (m.tiles_x * m.tiles_y);
+  const int64_t x0 = bx * S::kTX, y0 = by * S::kTY;
+  const int64_t z0 = bz * m.chunk;
+  const int64_t z1 = (z0 + m.chunk < g.n[2]) ? z0 + m.chunk : g.n[2];
+  const int64_t plane = g.n[0] * g.n[1];
+
+  // The thread's own cells: in-plane offsets (wrapped: cells past the
+  // grid's edge hold the wrapped neighbour, as the halo does), tile slots
+  // and store offsets (past the buffer's range outside the grid).
+  int64_t gown[CY][CX];
+  int lown[CY][CX], soff[CY][CX];
+#pragma unroll
+  for (int j = 0; j < CY; ++j)
+#pragma unroll
+    for (int i = 0; i < CX; ++i) {
+      const int64_t x = x0 + tx + BX * i, y = y0 + ty + BY * j;
+      gown[j][i] = filter_wrap(x, 0, g) + g.n[0] * filter_wrap(y, 1, g);
+      lown[j][i] = (ty + BY * j + R) * LX + tx + BX * i + R;
+      soff[j][i] = (x < g.n[0] && y < g.n[1]) ? (int)((x + g.n[0] * (y - y0)) * 8) : 0x7fffffff;
+    }
+  // This thread's share of the halo ring: the R rows above and below the
+  // tile (full width), then R columns left and right of each tile row.
+  int64_t ghal[NLH];
+  int lhal[NLH];
+#pragma unroll
+  for (int k = 0; k < NLH; ++k) {
+    const int e = tid + 256 * k;
+    int lx = 0, ly = 0;
+    if (e < 2 * R * LX) {
+      ly = e / LX;
+      lx = e - ly * LX;
+      if (ly >= R) ly += S::kTY;
+    } else if (e < NH) {
+      const int e2 = e - 2 * R * LX;
+      const int row = e2 / (2 * R), c = e2 - row * (2 * R);
+      ly = R + row;
+      lx = c < R ? c : LX - 2 * R + c;
+    }
+    ghal[k] = e < NH ? filter_wrap(x0 - R + lx, 0, g) + g.n[0] * filter_wrap(y0 - R + ly, 1, g)
+                     : 0;
+    lhal[k] = e < NH ? ly * LX + lx : NE + (tid & 63);
+  }
+  const int64_t tile_bytes = (g.n[1] - y0) * g.n[0] * 8;  // tile's first row to the plane's end
+  const int nrec = (int)(tile_bytes < 0x7fffffffLL ? tile_bytes : 0x7fffffffLL);
+
+  // Planes inside the loop are indexed 0 <= p < n2 + R <= 2 n2, where the
+  // reference's wrap is p or p - n2.
+  auto ld_own = [&](int64_t p, double (&v)[CY][CX]) {
+    const double* pl = in + (p < g.n[2] ? p : p - g.n[2]) * plane;
+#pragma unroll
+    for (int j = 0; j < CY; ++j)
+#pragma unroll
+      for (int i = 0; i < CX; ++i) v[j][i] = pl[gown[j][i]];
+  };
+  auto ld_hal = [&](int64_t p, double (&v)[NLH]) {
+    const double* pl = in + (p < g.n[2] ? p : p - g.n[2]) * plane;
+#pragma unroll
+    for (int k = 0; k < NLH; ++k) v[k] = pl[ghal[k]];
+  };
+
+  // Own cells of planes z-R .. z+R and of the prefetched plane z+R+1 in a
+  // circular queue of NQ = D + 1 slots, plane p in slot (p - z0 + R) mod NQ;
+  // the plane loop is unrolled NQ times (NQ is even), so every slot, halo
+  // buffer and LDS buffer index is a constant and nothing is copied.
+  constexpr int NQ = D + 1;
+  double q[NQ][CY][CX], h[2][NLH];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {  // (warm-up planes may be negative: the full wrap)
+    const double* pl = in + filter_wrap(z0 - R + d, 2, g) * plane;
+#pragma unroll
+    for (int j = 0; j < CY; ++j)
+#pragma unroll
+      for (int i = 0; i < CX; ++i) q[d][j][i] = pl[gown[j][i]];
+  }
+  ld_hal(z0, h[0]);
+#pragma unroll
+  for (int d = 0; d < D; ++d)
+#pragma unroll
+    for (int j = 0; j < CY; ++j)
+#pragma unroll
+      for (int i = 0; i < CX; ++i) settle(q[d][j][i]);
+#pragma unroll
+  for (int k = 0; k < NLH; ++k) settle(h[0][k]);
+
+  // Output plane z at phase U = (z - z0) mod NQ.
+  auto step = [&](int64_t z, auto phase) __attribute__((always_inline)) {
+    constexpr int U = decltype(phase)::value;
+    double* t = tile[U & 1];
+#pragma unroll
+    for (int j = 0; j < CY; ++j)
+#pragma unroll
+      for (int i = 0; i < CX; ++i) t[lown[j][i]] = q[(U + R) % NQ][j][i];
+#pragma unroll
+    for (int k = 0; k < NLH; ++k) t[lhal[k]] = h[U & 1][k];
+    lds_barrier();
+    // The next plane's loads (after the chunk's last plane: this one again).
+    const int64_t zn = z + 1 < z1 ? z + 1 : z;
+    ld_own(zn + R, q[(U + D) % NQ]);
+    ld_hal(zn, h[(U + 1) & 1]);
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(out + z * plane + y0 * g.n[0], 0, nrec, 0x00020000);
+#pragma unroll
+    for (int j = 0; j < CY; ++j) {
+#pragma unroll
+      for (int i = 0; i < CX; ++i) {
+        const int c = lown[j][i];
+        double acc = 0.0;
+        int u = 0;  // term index in map order
+#pragma unroll
+        for (int k = -R; k < 0; ++k) acc = acc + g.w[u++] * t[c + k];
+#pragma unroll
+        for (int k = -R; k < 0; ++k) acc = acc + g.w[u++] * t[c + k * LX];
+#pragma unroll
+        for (int k = -R; k <= R; ++k) acc = acc + g.w[u++] * q[(U + R + k + NQ) % NQ][j][i];
+#pragma unroll
+        for (int k = 1; k <= R; ++k) acc = acc + g.w[u++] * t[c + k * LX];
+#pragma unroll
+        for (int k = 1; k <= R; ++k) acc = acc + g.w[u++] * t[c + k];
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, acc), rs, soff[j][i], 0,
+                                              2 /* nt */);
+      }
+    }
+  };
+  int64_t z = z0;
+  for (; z + NQ <= z1; z += NQ) {
+    step(z, std::integral_constant<int, 0>{});
+    step(z + 1, std::integral_constant<int, 1>{});
+    step(z + 2, std::integral_constant<int, 2>{});
+    step(z + 3, std::integral_constant<int, 3>{});
+    if constexpr (NQ > 4) {
+      step(z + 4, std::integral_constant<int, 4 % NQ>{});
+      step(z + 5, std::integral_constant<int, 5 % NQ>{});
+    }
+  }
+  // The last planes (block-uniform branches).
+  if (z < z1) step(z, std::integral_constant<int, 0>{});
+  if (z + 1 < z1) step(z + 1, std::integral_constant<int, 1>{});
+  if (z + 2 < z1) step(z + 2, std::integral_constant<int, 2>{});
+  if constexpr (NQ > 4) {
+    if (z + 3 < z1) step(z + 3, std::integral_constant<int, 3>{});
+    if (z + 4 < z1) step(z + 4, std::integral_constant<int, 4 % NQ>{});
+  }
+}
+
+// Radius R if g is a full 3-D star stencil of radius R = 2 with its terms
+// in map order (see filter_star_kernel), else 0.
+int star_radius(const FilterGeom& g) {
+  for (int R = 2; R <= 2; ++R) {
+    if (g.nt != 6 * R + 1) continue;
+    int expect[kFilterMaxTerms][3] = {};
+    int u = 0;
+    for (int k = -R; k < 0; ++k) expect[u++][0] = k;
+    for (int k = -R; k < 0; ++k) expect[u++][1] = k;
+    for (int k = -R; k <= R; ++k) expect[u++][2] = k;
+    for (int k = 1; k <= R; ++k) expect[u++][1] = k;
+    for (int k = 1; k <= R; ++k) expect[u++][0] = k;
+    bool same = true;
+    for (int t = 0; t < g.nt && same; ++t)
+      for (int a = 0; a < 3; ++a) same = same && g.off[t][a] == expect[t][a];
+    if (same) return R;
+  }
+  return 0;
+}
+
+template <int R>
+bool star_plan(const FilterGeom& g, FilterMarch& m, int64_t& blocks) {
+  using S = StarShape<R>;
+  m = FilterMarch{};
+  m.rx = m.ry = m.rz = R;
+  m.LX = S::kLX;
+  m.LY = S::kLY;
+  m.D = 2 * R + 1;
+  // (n2 > 2R: the loop's planes p < n2 + R wrap to p or p - n2)
+  if (g.n[2] <= 2 * R || g.n[1] < S::kTY) return false;
+  if ((int64_t)S::kTY * g.n[0] * 8 >= 0x7fffffffLL) return false;  // store offsets
+  m.tiles_x = (g.n[0] + S::kTX - 1) / S::kTX;
+  m.tiles_y = (g.n[1] + S::kTY - 1) / S::kTY;
+  const int64_t tiles = m.tiles_x * m.tiles_y;
+  const int64_t cap =
+      kernel_capacity(reinterpret_cast<const void*>(&filter_star_kernel<R>), 256, 0);
+  const int64_t per_round = std::max<int64_t>(1, cap / tiles);
+  const int64_t nch = march_chunks(tiles, g.n[2], 2 * R, cap,
+                                   std::max<int64_t>(1, per_round / 2), 4 * per_round, true);
+  m.chunk = (g.n[2] + nch - 1) / nch;
+  blocks = tiles * ((g.n[2] + m.chunk - 1) / m.chunk);
+  return blocks > 0 && blocks <= 0x7fffffffLL;
+}
+
 template <int NT, class S>
 const void* march_kernel_ptr() {
   return reinterpret_cast<const void*>(&filter_march_kernel<NT, S>);
@@ -344,7 +579,8 @@ __global__ __launch_bounds__(kFilterThreads) void filter_sin_kernel(double* __re
 hipError_t plan_filter(const FilterGeom& g, FilterPlan* p) {
   // 2-D grids (no axis-2 extent or offsets): rows marched along axis 1,
   // the grid viewed as n0 x 1 x n1 (same memory order).  3-D grids with
-  // n1 >= 8: planes marched along axis 2 (rings up to 160 KB of LDS).
+  // n1 >= 8: full star stencils of radius 2 by the star kernel, others by
+  // planes marched along axis 2 (rings up to 160 KB of LDS).
   // Otherwise (1-D grids, stencils wider than the ring allows): the
   // gather kernel.
   *p = FilterPlan{};
@@ -367,6 +603,13 @@ hipError_t plan_filter(const FilterGeom& g, FilterPlan* p) {
       p->g = h;
       return hipSuccess;
     }
+  }
+  const int star = g.n[2] > 1 ? star_radius(g) : 0;
+  if (star == 2 && star_plan<2>(g, p->m, p->blocks)) {
+    p->kind = kFilterStar;
+    p->g = g;
+    p->lds = 0;  // static
+    return hipSuccess;
   }
   if (g.n[1] >= March3D::kTY && march_plan<March3D>(g, p->m, p->lds, p->blocks)) {
     p->kind = kFilterPlanes;
@@ -397,6 +640,11 @@ hipError_t launch_filter(const FilterPlan& p, const double* in, double* out, hip
   if (p.kind == kFilterPlanes) return launch_march<March3D>(p, in, out, s);
   if (p.kind == kFilterPlanesMid) return launch_march<March3DMid>(p, in, out, s);
   if (p.kind == kFilterPlanesWide) return launch_march<March3DWide>(p, in, out, s);
+  if (p.kind == kFilterStar) {
+    const dim3 grid((unsigned)p.blocks), block(64, 4);
+    hipLaunchKernelGGL(filter_star_kernel<2>, grid, block, 0, s, in, out, p.g, p.m);
+    return hipGetLastError();
+  }
   const FilterGeom& g = p.g;
   const int64_t xblocks = (g.n[0] + kFilterThreads - 1) / kFilterThreads;
   const dim3 grid((unsigned)p.blocks), block(kFilterThreads);

@@ -148,7 +148,7 @@ struct FilterMarch {
 
 // How one filter application is launched, fixed once per geometry.
 struct FilterPlan {
-  int kind;      // kFilterGather, kFilterPlanes(Mid, Wide) or kFilterRows
+  int kind;      // kFilterGather, kFilterPlanes(Mid, Wide), kFilterRows or kFilterStar
   FilterGeom g;  // geometry as launched (kFilterRows: viewed as n0 x 1 x n1)
   FilterMarch m;
   size_t lds;    // dynamic LDS bytes per block
@@ -159,6 +159,7 @@ constexpr int kFilterPlanes = 1;  // 3-D: 128 x 8 tiles marched along axis 2
 constexpr int kFilterRows = 2;    // 2-D: 512-wide tiles marched along axis 1
 constexpr int kFilterPlanesWide = 3;  // 3-D, halo tiles up to radius 8
 constexpr int kFilterPlanesMid = 4;   // 3-D, 128 x 4 tiles, radius 2..3
+constexpr int kFilterStar = 5;        // 3-D full star stencils of radius 2 (m.rz = 2)
 
 // Picks the kernel and tiling for g (needs the device current).
 hipError_t plan_filter(const FilterGeom& g, FilterPlan* p);

@@ -88,6 +88,13 @@ class Filter:
         return True
 
     # -- extensions --------------------------------------------------------
+    def kernel(self) -> str:
+        """The kernel applyFilter launches ("star", "planes", "planes_mid",
+        "planes_wide", "rows" or "gather"; fixed at creation)."""
+        name = C.c_char_p()
+        L.check(L.lib().fdb_filter_kernel(self._h, C.byref(name)))
+        return name.value.decode()
+
     def iterate(self, niters: int) -> float:
         """niters x (applyFilter(); copyOutToIn()); returns kernel ms."""
         ms = C.c_float()

@@ -73,9 +73,11 @@ def compare(a: torch.Tensor, b: torch.Tensor) -> dict:
 
 def misfit_gradient(w_eval: Workload, target: torch.Tensor, f_dtype: int, lam_dtype: int,
                     max_bytes: int, f0: torch.Tensor | None = None,
-                    method: str = "checkpoint") -> dict:
+                    method: str = "checkpoint", reps: int = 1) -> dict:
     """Gradient of the misfit at w_eval's velocity and field, by the tape's
-    checkpointed or shift-invariant method."""
+    checkpointed or shift-invariant method.  With reps > 1 the forward,
+    misfit and backward run reps times on one tape (same results) and
+    `seconds` is the median; `seconds_all` lists every run."""
     if f0 is None:
         f0 = init_field(w_eval, f_dtype)
     torch.cuda.synchronize()
@@ -83,15 +85,20 @@ def misfit_gradient(w_eval: Workload, target: torch.Tensor, f_dtype: int, lam_dt
     tape = Tape(w_eval.num_cells, w_eval.velocity, w_eval.lengths, w_eval.dt, w_eval.steps,
                 f_dtype, lam_dtype, max_bytes, method=method)
     t1 = time.perf_counter()
-    fT = torch.empty_like(f0)
-    tape.forward(f0, fT)
-    loss, lamT = misfit(fT, target, lam_dtype)
-    lam0 = torch.empty_like(lamT)
-    grad_v = tape.backward(lamT, lam0)
-    torch.cuda.synchronize()
-    t2 = time.perf_counter()
+    runs = []
+    for _ in range(max(1, reps)):
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        fT = torch.empty_like(f0)
+        tape.forward(f0, fT)
+        loss, lamT = misfit(fT, target, lam_dtype)
+        lam0 = torch.empty_like(lamT)
+        grad_v = tape.backward(lamT, lam0)
+        torch.cuda.synchronize()
+        runs.append(time.perf_counter() - ta)
+        del fT, lamT
     st = tape.stats()
     tape.close()
-    del fT, lamT
-    return {"loss": loss, "grad_f0": lam0, "grad_v": grad_v, "seconds": t2 - t1,
+    return {"loss": loss, "grad_f0": lam0, "grad_v": grad_v,
+            "seconds": sorted(runs)[len(runs) // 2], "seconds_all": runs,
             "alloc_seconds": t1 - t0, **st}

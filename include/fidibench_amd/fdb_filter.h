@@ -53,6 +53,12 @@ int fdb_filter_copy_out_to_in(fdb_filter_t* h);
  * laplacian.cxx:80-86 -- with the kernels' time (HIP events) in *ms. */
 int fdb_filter_iterate_timed(fdb_filter_t* h, int niters, float* ms);
 
+/* The kernel this filter's applyFilter launches, fixed at creation:
+ * "star" (full 3-D star stencils of radius 2: the 4th-order
+ * Laplacian), "planes", "planes_mid", "planes_wide" (3-D LDS rings),
+ * "rows" (2-D) or "gather".  Extension (no counterpart in the reference). */
+int fdb_filter_kernel(const fdb_filter_t* h, const char** name);
+
 /* computeCheckSum("input" | "output"), Filter.cpp:466-485: which = 0 for
  * input, 1 for output; a sequential left fold on the host. */
 int fdb_filter_checksum(fdb_filter_t* h, int which, double* out);

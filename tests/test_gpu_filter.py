@@ -66,6 +66,49 @@ CASES = [((100,), "lap"), ((64, 48), "lap"), ((6, 10), "lap"), ((33, 8, 20), "la
          ((70, 9, 11), "lap4"), ((300, 40), "lap4")]
 
 
+def _star(rng, R):
+    """A full 3-D star of radius R with random weights (the term order in
+    std::map order pins which weight goes with which neighbour)."""
+    st = {(0, 0, 0): float(rng.standard_normal())}
+    for a in range(3):
+        for k in range(1, R + 1):
+            for sg in (1, -1):
+                o = [0, 0, 0]
+                o[a] = sg * k
+                st[tuple(o)] = float(rng.standard_normal())
+    return st
+
+
+# Full 3-D stars of radius 2 take the star kernel (128 x 8 tiles, own
+# cells queued in registers, one halo plane in LDS): partial tiles on both
+# axes, grids narrower than a tile (12 x 8), sizes whose negative-index wrap
+# is not periodic (non-powers of two), and n2 = 5, the smallest it takes;
+# n2 = 4 goes to the ring kernel, as do radius-1 stars (random weights).
+STAR_CASES = [((257, 9, 5), 2, "star"), ((128, 16, 64), 2, "star"), ((12, 8, 9), 2, "star"),
+              ((1000, 24, 7), 2, "star"), ((130, 17, 6), 2, "star"),
+              ((64, 16, 4), 2, "planes_mid"), ((130, 17, 6), 1, "planes"),
+              ((40, 8, 3), 1, "planes"), ((30, 7, 9), 1, "planes_mid")]
+
+
+@pytest.mark.parametrize("dims,R,kernel", STAR_CASES)
+def test_star_stencils_bit_exact_vs_oracle(oracle, dims, R, kernel):
+    from fidibench_amd import Filter
+    rng = np.random.default_rng(sum(dims) + R)
+    st = _star(rng, R)
+    f0 = rng.standard_normal(int(np.prod(dims)))
+    flt = Filter(dims, [0.0] * 3, [1.0] * 3, st)
+    assert flt.kernel() == kernel
+    flt.setInData(f0)
+    ref = f0
+    for it in range(3):
+        flt.applyFilter()
+        ref = oracle.filter_apply(ref, dims, list(st.items()))
+        got = flt.get_data("output").ravel(order="F")
+        assert np.array_equal(bits(got), bits(ref)), (dims, R, it)
+        flt.copyOutToIn()
+    flt.close()
+
+
 @pytest.mark.parametrize("dims,kind", CASES)
 def test_filter_bit_exact_vs_oracle(oracle, dims, kind):
     from fidibench_amd import Filter
