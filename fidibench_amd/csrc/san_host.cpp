@@ -1,4 +1,4 @@
-// san_host.cpp -- CPU driver of the library's host code under the address
+// san_host.cpp -- CPU test program of the library's host code under the address
 // and undefined-behaviour sanitizers (test infrastructure: built by
 // `make -C fidibench_amd/csrc sanitize` with the host side of every source
 // instrumented, run by tests/test_host_sanitize.py; never shipped).

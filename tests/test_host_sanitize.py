@@ -1,11 +1,11 @@
 """The library's host code under AddressSanitizer and UndefinedBehaviorSanitizer
 on the CPU: `make -C fidibench_amd/csrc sanitize` builds
 fidibench_amd/csrc/san_host.cpp with every source's host side instrumented,
-and the driver exercises the checkpoint schedule (T up to 400), the
+and the test program exercises the checkpoint schedule (T up to 400), the
 reference checksum/std folds, both VTK writers, the raw-file header checks
 (good, truncated, foreign, malformed, overflowing axes), the CLI parser,
 constructor argument validation and the tuning-variable parser.  Any
-sanitizer report aborts the driver; leak detection is on."""
+sanitizer report aborts the program; leak detection is on."""
 from __future__ import annotations
 
 import os
