@@ -62,12 +62,14 @@ template <> struct St<uint16_t> {
 // Stochastic rounding of step results for bf16 storage (FDB_BF16_SR): 16
 // random bits are added below the kept bits of the binary32 value, then
 // truncated, so x rounds up with probability equal to the discarded
-// fraction (unbiased).  The bits are one half of a pair word: lowbias32
-// (C. Wellons' 32-bit integer hash) of the index of the cell's pair (flat
-// index / 2), xor a per-step key made on the host from the seed and the
-// step's index, through one multiply-xorshift round (sr_round): the low
-// half for an even cell, the high half for an odd one (one word per two
-// cells).  The pair's lowbias32 does not depend on the step, so a fused
+// fraction (unbiased).  Cells pair along axis 1: the pair of (i0, i1, i2)
+// is the cell (i0, i1 & ~1, i2), named by its flat index.  The bits are
+// one half of a pair word: lowbias32 (C. Wellons' 32-bit integer hash) of
+// the pair index, xor a per-step key made on the host from the seed and
+// the step's index, through one multiply-xorshift round (sr_round): the
+// low half for the even row, the high half for the odd one (one word per
+// two cells; a thread or lane holds both rows of a pair, so no exchange is
+// needed).  The pair's lowbias32 does not depend on the step, so a fused
 // pass hashes each pair once per plane and pays one multiply per pair and
 // step.  Inf stays Inf, NaN stays quiet.
 __host__ __device__ inline uint32_t lowbias32(uint32_t x) {
@@ -119,19 +121,13 @@ __device__ inline uint16_t bf16_sr_bits(float x, uint32_t r16) {
                                       __builtin_amdgcn_cvt_sr_bf16_f32(z, x, r16 << 16, false));
 }
 
-// Two values of one pair rounded with the pair's hash h: the even cell
-// with its low 16 bits, the odd cell with its high 16 bits; packed low =
-// even.  Two conversions and a shift.
-__device__ inline uint32_t bf16x2_sr(float even, float odd, uint32_t h) {
+// Two values of one row rounded with their pairs' words h0, h1 (the even
+// row takes the low 16 bits, the odd row the high 16); packed low = first.
+__device__ inline uint32_t bf16x2_sr(float a, float b, uint32_t h0, uint32_t h1, bool odd) {
   s16x2 v = {0, 0};
-  v = __builtin_amdgcn_cvt_sr_bf16_f32(v, even, h << 16, false);
-  v = __builtin_amdgcn_cvt_sr_bf16_f32(v, odd, h, true);
+  v = __builtin_amdgcn_cvt_sr_bf16_f32(v, a, odd ? h0 : h0 << 16, false);
+  v = __builtin_amdgcn_cvt_sr_bf16_f32(v, b, odd ? h1 : h1 << 16, true);
   return __builtin_bit_cast(uint32_t, v);
-}
-
-__device__ inline uint16_t bf16_sr(float x, uint64_t cell, uint32_t key) {
-  const uint32_t h = sr_pair_hash(cell >> 1, key);
-  return bf16_sr_bits(x, (cell & 1) ? (h >> 16) : (h & 0xffffu));
 }
 
 typedef float f32x2 __attribute__((ext_vector_type(2)));
@@ -518,22 +514,20 @@ __device__ inline void store_pack_nt(TS* p, const typename St<TS>::M (&in)[VEC])
 }
 
 // bf16 store with stochastic rounding (cell = flat index of the first).
+// VEC consecutive cells of one row rounded with their pairs' words h[v]
+// (odd: the row is the odd one of its pairs).
 template <int VEC, bool NT>
-__device__ inline void store_pack_sr(uint16_t* p, const float (&in)[VEC], uint64_t cell,
-                                     uint32_t key) {
+__device__ inline void store_pack_sr(uint16_t* p, const float (&in)[VEC],
+                                     const uint32_t (&h)[VEC], bool odd) {
   Pack<uint16_t, VEC> pk;
   if constexpr (VEC % 2 == 0) {
-    // cell is even here (n2 % VEC == 0 and the pack starts a VEC group):
-    // one hash serves the pair (cell + v, cell + v + 1).
-    const uint64_t pair0 = cell >> 1;
     uint32_t w[VEC / 2];
 #pragma unroll
-    for (int v = 0; v < VEC; v += 2)
-      w[v / 2] = bf16x2_sr(in[v], in[v + 1], sr_pair_hash(pair0 + (uint64_t)(v / 2), key));
+    for (int v = 0; v < VEC; v += 2) w[v / 2] = bf16x2_sr(in[v], in[v + 1], h[v], h[v + 1], odd);
     pk = __builtin_bit_cast(Pack<uint16_t, VEC>, w);
   } else {
 #pragma unroll
-    for (int v = 0; v < VEC; ++v) pk.v[v] = bf16_sr(in[v], cell + v, key);
+    for (int v = 0; v < VEC; ++v) pk.v[v] = bf16_sr_bits(in[v], odd ? h[v] >> 16 : h[v] & 0xffffu);
   }
   using R = typename RawT<sizeof(Pack<uint16_t, VEC>)>::T;
   const R raw = __builtin_bit_cast(R, pk);
@@ -672,6 +666,7 @@ __device__ inline void rows_tile(const TS* __restrict__ in, TS* __restrict__ out
       }
     }
     TS* obase = out + p * plane + i2c;
+    uint32_t hw[SR ? VEC : 1];  // SR: the pair words of the current row pair
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       M o[VEC];
@@ -698,12 +693,19 @@ __device__ inline void rows_tile(const TS* __restrict__ in, TS* __restrict__ out
         }
         prev[r][v] = x;
       }
-      if (col_ok && r < nvalid) {
-        if constexpr (SR)
-          store_pack_sr<VEC, NT>(obase + roff[r], o, (uint64_t)(p * plane + roff[r] + i2c),
-                                 a.sr_key);
-        else
-          store_pack_nt<TS, VEC, NT>(obase + roff[r], o);
+      if constexpr (SR) {
+        // Pair words of this row's cells, shared with the row's partner
+        // when the thread holds both (computed at the pair's first row).
+        const int64_t grow = r0 + r;
+        const bool odd = (grow & 1) != 0;
+        if (!odd || r == 0) {
+          const uint64_t base = (uint64_t)(p * plane + (grow - (odd ? 1 : 0)) * a.n2 + i2c);
+#pragma unroll
+          for (int v = 0; v < VEC; ++v) hw[v] = sr_pair_hash(base + (uint64_t)v, a.sr_key);
+        }
+        if (col_ok && r < nvalid) store_pack_sr<VEC, NT>(obase + roff[r], o, hw, odd);
+      } else if (col_ok && r < nvalid) {
+        store_pack_nt<TS, VEC, NT>(obase + roff[r], o);
       }
     }
     if constexpr (ADJ) {
@@ -833,11 +835,11 @@ struct TbArgs {
 //   kTbSR        K forward steps in bf16 storage, each level rounded
 //                stochastically with the bits a one-step launch of that step
 //                draws (sr_pair_hash of the cell's pair and the step's key,
-//                low half for an even cell, high half for an odd one),
-//                so the result is the one-step launches' bit for bit.  The
-//                two lanes of a cell pair (even row length, even K: lane
-//                parity = column parity) split the hashing: each hashes half
-//                the rows and they swap results with a DPP quad permutation.
+//                low half for the even row, high half for the odd one),
+//                so the result is the one-step launches' bit for bit.  A
+//                lane holds both rows of each of its pairs (even rows per
+//                wave, even K, even n1), so each lane hashes one word per
+//                pair of rows and level, with no exchange between lanes.
 constexpr int kTbPlain = 0, kTbStoreAll = 1, kTbAdjoint = 2, kTbSR = 3;
 struct TbAux {
   const void* fs[kMaxTimeBlock];
@@ -927,15 +929,13 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
     roff[r] = pmod(rl, a.n1) * a.n2;
     row_out[r] = (LO1 ? br >= K : br < WY * R - K) && rl >= 0 && rl < a.n1;
   }
-  // SR: this lane hashes rows 2j (even lane) or 2j+1 (odd lane) of its
-  // column's pairs; sword[j] is that cell's pair word (sr_fold) for the current plane.
+  // SR: rows 2j and 2j+1 of the wave are the two rows of a pair (the
+  // wave's first row is even: R, K and the tile's row count are even, and
+  // an even n1 keeps the parity of wrapped rows); sword[j] is the pair
+  // word (sr_fold) of this lane's column for the current plane.
   constexpr int J = (R + 1) / 2;
-  const bool odd = (lane & 1) != 0;
+  static_assert(!SR || (R % 2 == 0 && K % 2 == 0), "SR passes pair whole rows");
   uint32_t sword[SR ? J : 1];
-  // SR: v_perm_b32 selectors (own hash = bytes 7..4, partner's = 3..0) that
-  // put the cell's 16 rounding bits in bits 31:16 (low bytes zero), for an
-  // even row (the even lane's own hash) and an odd row (the odd lane's own).
-  const uint32_t ssel[2] = {odd ? 0x03020c0cu : 0x05040c0cu, odd ? 0x07060c0cu : 0x01000c0cu};
   // Planes: output p_lo .. p_hi-1; march K planes earlier (upwind side).
   const int64_t plane = a.n1 * a.n2;
   const int64_t p_lo = ch * a.chunk;
@@ -1004,9 +1004,7 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
     if constexpr (SR) {
 #pragma unroll
       for (int j = 0; j < J; ++j) {
-        // (row offsets are uniform: the lane's row is picked per plane)
-        const int64_t ro = odd ? roff[2 * j + 1 < R ? 2 * j + 1 : R - 1] : roff[2 * j];
-        const uint64_t pair = ((uint64_t)(pw * plane + ro) + col) >> 1;
+        const uint64_t pair = (uint64_t)(pw * plane + roff[2 * j]) + col;
         sword[j] = sr_fold(sr_pair_word(pair));  // once per plane, for all K levels
       }
     }
@@ -1144,7 +1142,7 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
         d1[r] = cf.c[1] * d1[r];
         d2[r] = cf.c[2] * d2[r];
       }
-      uint32_t own = 0, par = 0;  // SR: the hashes of rows 2j, 2j+1 (this lane's, its partner's)
+      uint32_t word = 0;  // SR: the pair word of rows 2j, 2j+1
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         M g = x[r];
@@ -1154,18 +1152,13 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
         const M gone = prev[l][r];  // SR: a bf16 value, so its low 16 bits are zero
         prev[l][r] = x[r];
         if constexpr (SR) {
-          // The pair's hash (computed by the lane of the row's parity, at
-          // the row pair's first row, so only two are live); the even cell
-          // takes its low half, the odd cell its high half, which
-          // v_cvt_sr_bf16_f32 reads from bits 31:16 of its operand: one
-          // v_perm_b32 puts the right half of the right hash there.  The
+          // The pair's word (at the pair's first row, for both): the even
+          // row takes its low half, the odd row its high half, which
+          // v_cvt_sr_bf16_f32 reads from bits 31:16 of its operand.  The
           // conversion writes the high half of a register whose low half is
           // kept, so it is given a dead bf16 value (low half zero).
-          if (r % 2 == 0) {
-            own = sr_round_pre(sword[r / 2] ^ a.sr_key[l]);
-            par = __builtin_amdgcn_mov_dpp(own, 0xb1, 0xf, 0xf, true);
-          }
-          const uint32_t rbits = __builtin_amdgcn_perm(own, par, ssel[r & 1]);
+          if (r % 2 == 0) word = sr_round_pre(sword[r / 2] ^ a.sr_key[l]);
+          const uint32_t rbits = (r % 2 == 0) ? word << 16 : word;
           x[r] = __builtin_bit_cast(
               float, __builtin_amdgcn_cvt_sr_bf16_f32(__builtin_bit_cast(s16x2, gone), g,
                                                       rbits, true));
@@ -1578,14 +1571,15 @@ TbCfg tb_cfg(int bytes, int K) { return tuning().tb[bytes][K]; }
 // levels != nullptr: also store levels 1..K-1 to levels[0..K-2] (K = 4
 // or 8, FP64 only: with one 4-byte cell per lane the FP32 kernel writes its
 // 8 levels at ~2 TB/s, slower than one-step launches storing 16 B per lane).
-// Launch shape of the stochastic-rounding bf16 pass (K = 2, 4 or 8: even,
-// so lane pairs hold cell pairs): within 128 VGPRs, i.e. four waves per
-// SIMD like the round-to-nearest pass -- at K = 8 with 8 rows per wave
-// (9 need 132 VGPRs).
-TbCfg tb_sr_cfg(int K) { return K == 8 ? TbCfg{8, 8, 0} : TbCfg{9, 4, 0}; }
+// Launch shape of the stochastic-rounding bf16 pass (K = 2, 4 or 8): an
+// even number of rows per wave and an even K, so the wave's rows start at
+// an even row and hold whole pairs; within 128 VGPRs, i.e. four waves per
+// SIMD like the round-to-nearest pass.
+TbCfg tb_sr_cfg(int K) { return K == 8 ? TbCfg{8, 8, 0} : TbCfg{8, 4, 0}; }
 
+// The fused SR pass needs an even n1 (wrapped rows keep their parity).
 bool tb_sr_ok(const Grid3& g, int K) {
-  return (K == 2 || K == 4 || K == 8) && g.n[2] % 2 == 0;
+  return (K == 2 || K == 4 || K == 8) && g.n[1] % 2 == 0;
 }
 
 template <typename TS>
@@ -1652,8 +1646,8 @@ hipError_t tb_dispatch(const Grid3& g, const TS* in, TS* out, int K, hipStream_t
     FDB_TB(9, 8, 8, kTbStoreAll)
   }
   if constexpr (sizeof(TS) == 2) {
-    FDB_TB(9, 2, 4, kTbSR)
-    FDB_TB(9, 4, 4, kTbSR)
+    FDB_TB(8, 2, 4, kTbSR)
+    FDB_TB(8, 4, 4, kTbSR)
     FDB_TB(8, 8, 8, kTbSR)
   }
 #undef FDB_TB

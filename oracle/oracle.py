@@ -49,8 +49,8 @@ def lib() -> C.CDLL:
             "orc_advect_bf16": (C.c_int, [C.c_int, _u64p, _dp, _dp, C.c_double, C.c_int, _u16p]),
             "orc_advect_bf16_sr": (C.c_int, [C.c_int, _u64p, _dp, _dp, C.c_double, C.c_int,
                                              C.c_uint64, C.c_uint64, _u16p]),
-            "orc_bf16_sr": (C.c_uint16, [C.c_float, C.c_uint64, C.c_uint32]),
-            "orc_sr_bits": (C.c_uint32, [C.c_uint64, C.c_uint32]),
+            "orc_bf16_sr": (C.c_uint16, [C.c_float, C.c_uint64, C.c_int, C.c_uint32]),
+            "orc_sr_bits": (C.c_uint32, [C.c_uint64, C.c_int, C.c_uint32]),
             "orc_sr_step_key": (C.c_uint32, [C.c_uint64, C.c_uint64]),
             "orc_adjoint_f64": (C.c_int, [C.c_int, _u64p, _dp, _dp, C.c_double, C.c_int, _dp]),
             "orc_adjoint_step_lowp": (C.c_int, [C.c_int, _u64p, _dp, _dp, C.c_double, C.c_int,

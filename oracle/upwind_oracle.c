@@ -213,11 +213,13 @@ int orc_advect_bf16(int nd, const uint64_t* n, const double* v, const double* le
 
 /* bfloat16 storage with stochastic rounding of every step's result: 16
  * random bits added below the kept bits of the binary32 value, then
- * truncated.  The bits come from lowbias32 (C. Wellons' integer hash) of
- * the folded index of the cell's pair (cell / 2), xor a per-step key hashed
- * from (seed, step), through one multiply-xorshift round (lowbias32's first
- * round and closing shift): the low half for an even cell, the high half
- * for an odd one. */
+ * truncated.  Cells pair along the second-to-last axis (axis 1 of the grid
+ * padded to three axes): a cell's pair is the flat index of the cell with
+ * that index rounded down to even (1-D grids: the cell itself).  The bits
+ * come from lowbias32 (C. Wellons' integer hash) of the folded pair index,
+ * xor a per-step key hashed from (seed, step), through one multiply-
+ * xorshift round (lowbias32's first round and closing shift): the low half
+ * for the even row, the high half for the odd one. */
 static uint32_t orc_lowbias32(uint32_t x) {
   x ^= x >> 16;
   x *= 0x7feb352du;
@@ -238,18 +240,17 @@ static uint32_t orc_sr_round(uint32_t x) {
   return x ^ (x >> 16);
 }
 
-uint32_t orc_sr_bits(uint64_t cell, uint32_t key) {
-  const uint64_t pr = cell >> 1;
-  const uint32_t h = orc_sr_round(orc_lowbias32((uint32_t)(pr ^ (pr >> 32))) ^ key);
-  return (cell & 1u) ? (h >> 16) : (h & 0xffffu);
+uint32_t orc_sr_bits(uint64_t pair, int odd, uint32_t key) {
+  const uint32_t h = orc_sr_round(orc_lowbias32((uint32_t)(pair ^ (pair >> 32))) ^ key);
+  return odd ? (h >> 16) : (h & 0xffffu);
 }
 
-uint16_t orc_bf16_sr(float x, uint64_t cell, uint32_t key) {
+uint16_t orc_bf16_sr(float x, uint64_t pair, int odd, uint32_t key) {
   uint32_t b;
   memcpy(&b, &x, 4);
   if ((b & 0x7f800000u) == 0x7f800000u)
     return (uint16_t)((b >> 16) | ((b & 0x007fffffu) ? 0x0040u : 0u));
-  return (uint16_t)((b + orc_sr_bits(cell, key)) >> 16);
+  return (uint16_t)((b + orc_sr_bits(pair, odd, key)) >> 16);
 }
 
 /* As orc_advect_bf16, with step number step0 + s rounded stochastically. */
@@ -272,7 +273,10 @@ int orc_advect_bf16_sr(int nd, const uint64_t* n, const double* v, const double*
         uint64_t up = orc_shift(&g, (uint64_t)i, j, g.up[j]);
         fi = fi - cf[j] * (old[up] - old[i]);
       }
-      f[i] = orc_bf16_sr(fi, (uint64_t)i, key);
+      // the pair along the second-to-last axis
+      const uint64_t st = nd >= 2 ? g.stride[nd - 2] : 1;
+      const int odd = nd >= 2 ? (int)(((uint64_t)i / st) % g.n[nd - 2] & 1u) : 0;
+      f[i] = orc_bf16_sr(fi, (uint64_t)i - (uint64_t)odd * st, odd, key);
     }
   }
   free(old);

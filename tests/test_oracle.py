@@ -211,29 +211,31 @@ def test_bf16_stochastic_rounding_oracle(oracle):
     lo = oracle.f32_to_bf16(np.array([np.float32(1.0)]))[0]
     hi = lo + 1
     key = L.orc_sr_step_key(3, 9)
-    outs = np.array([L.orc_bf16_sr(float(x), i, key) for i in range(20000)], dtype=np.uint16)
+    outs = np.array([L.orc_bf16_sr(float(x), i >> 1, i & 1, key) for i in range(20000)],
+                    dtype=np.uint16)
     assert set(np.unique(outs)) <= {lo, hi}
     mean = oracle.bf16_to_f32(outs).astype(np.float64).mean()
     assert abs(mean - float(x)) < 2e-5  # unbiased: E[round(x)] = x
     for v in (1.0, -2.5, 0.0, 3.0e38):
         b = oracle.f32_to_bf16(np.array([v], dtype=np.float32))[0]
         exact = float(oracle.bf16_to_f32(np.array([b]))[0])
-        assert L.orc_bf16_sr(exact, 12345, key) == b
-    assert L.orc_bf16_sr(float("inf"), 1, key) == 0x7F80
-    assert (L.orc_bf16_sr(float("nan"), 1, key) & 0x7FC0) == 0x7FC0
-    # Bit layout: one hash per cell pair, low half for the even cell.
-    for cell in (0, 1, 2, 3, 12345, 12346, 2 ** 33 + 7, 2 ** 40 + 2):
-        pr = cell >> 1
-        h = _sr_round(_lowbias32((pr ^ (pr >> 32)) & 0xFFFFFFFF) ^ key)
-        r = (h >> 16) if cell & 1 else (h & 0xFFFF)
-        assert L.orc_sr_bits(cell, key) == r
-        b = int(np.float32(1.3).view(np.uint32))
-        assert L.orc_bf16_sr(1.3, cell, key) == ((b + r) >> 16) & 0xFFFF
+        assert L.orc_bf16_sr(exact, 12345, 1, key) == b
+    assert L.orc_bf16_sr(float("inf"), 1, 0, key) == 0x7F80
+    assert (L.orc_bf16_sr(float("nan"), 1, 1, key) & 0x7FC0) == 0x7FC0
+    # Bit layout: one hash per cell pair, low half for the even row.
+    for pr in (0, 1, 12345, 2 ** 33 + 7, 2 ** 40 + 2):
+        for odd in (0, 1):
+            h = _sr_round(_lowbias32((pr ^ (pr >> 32)) & 0xFFFFFFFF) ^ key)
+            r = (h >> 16) if odd else (h & 0xFFFF)
+            assert L.orc_sr_bits(pr, odd, key) == r
+            b = int(np.float32(1.3).view(np.uint32))
+            assert L.orc_bf16_sr(1.3, pr, odd, key) == ((b + r) >> 16) & 0xFFFF
 
 
 def _sr_words(L, cells, steps, seed=7):
-    return np.array([[L.orc_sr_bits(int(c), L.orc_sr_step_key(seed, t)) for c in cells]
-                     for t in steps], dtype=np.float64) / 65536.0
+    """Rounding fractions of 'cells' c = the two halves (c & 1) of pair c >> 1."""
+    return np.array([[L.orc_sr_bits(int(c) >> 1, int(c) & 1, L.orc_sr_step_key(seed, t))
+                      for c in cells] for t in steps], dtype=np.float64) / 65536.0
 
 
 def test_bf16_sr_bits_are_uniform_and_uncorrelated(oracle):
@@ -291,6 +293,39 @@ def test_bf16_sr_advect_oracle(oracle):
     h = oracle.advect_bf16_sr(g0, (0.3, -0.2, 0.4), (1, 1, 1), 0.01, 2, seed=1)
     h = oracle.advect_bf16_sr(h, (0.3, -0.2, 0.4), (1, 1, 1), 0.01, 3, seed=1, step0=2)
     assert np.array_equal(a, h)
+
+
+def test_bf16_sr_pairs_along_axis_1(oracle):
+    """One SR step of the oracle equals a Python restatement that pairs the
+    cells along the second-to-last axis (rows 2m and 2m+1 share a word: the
+    low half for the even row), for 3-D, 2-D and 1-D grids and an odd
+    row count (the last row's partner does not exist)."""
+    L = oracle.lib()
+    rng = np.random.default_rng(11)
+    for shape, v in (((3, 5, 4), (0.3, -0.2, 0.4)), ((6, 7), (0.25, -0.5)), ((9,), (0.5,))):
+        g0 = oracle.f32_to_bf16(rng.random(shape, dtype=np.float32)).reshape(shape)
+        nd = len(shape)
+        dt = 0.05
+        got = oracle.advect_bf16_sr(g0, v, (1,) * nd, dt, 1, seed=5, step0=3)
+        f = oracle.bf16_to_f32(g0).astype(np.float32).ravel()
+        lens = np.ones(nd)
+        up = [1 if vj < 0 else -1 for vj in v]
+        c = [np.float32(np.float64(dt) * v[j] * up[j] / (lens[j] / shape[j])) for j in range(nd)]
+        key = L.orc_sr_step_key(5, 3)
+        exp = np.empty(f.size, dtype=np.uint16)
+        strides = [int(np.prod(shape[j + 1:])) for j in range(nd)]
+        for i in range(f.size):
+            idx = [(i // strides[j]) % shape[j] for j in range(nd)]
+            fi = f[i]
+            for j in range(nd):
+                nb = list(idx)
+                nb[j] = (idx[j] + up[j] + shape[j]) % shape[j]
+                k = sum(a * b for a, b in zip(nb, strides))
+                fi = np.float32(fi - np.float32(c[j] * np.float32(f[k] - f[i])))
+            odd = (idx[nd - 2] & 1) if nd >= 2 else 0
+            pair = i - odd * (strides[nd - 2] if nd >= 2 else 0)
+            exp[i] = L.orc_bf16_sr(float(fi), pair, odd, key)
+        assert np.array_equal(got.ravel(), exp), shape
 
 
 def test_lowp_adjoint_step_oracle(oracle):
