@@ -36,7 +36,8 @@ typedef struct fdb_upwind fdb_upwind_t;
  * it; F32 and BF16 store the field in 4 and 2 bytes and compute in float.
  * BF16_SR is bf16 storage whose step results are rounded stochastically
  * (unbiased; the random bits are a hash of the seed, the step's index and
- * the cell's flat index, so runs are reproducible).  Host conversions into
+ * the index of the cell's pair -- cells pair along the second-to-last axis
+ * -- so runs are reproducible).  Host conversions into
  * BF16 and BF16_SR fields round to nearest even. */
 enum fdb_dtype { FDB_F64 = 0, FDB_F32 = 1, FDB_BF16 = 2, FDB_BF16_SR = 3 };
 
