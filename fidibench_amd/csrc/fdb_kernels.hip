@@ -901,7 +901,13 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
   __shared__ M edge[2][WY][64];
   int64_t t = blockIdx.x;
   const int64_t nb = gridDim.x;
-  if ((nb & 7) == 0) t = (t & 7) * (nb >> 3) + (t >> 3);  // XCD-contiguous tiles
+  // Hardware hands blocks to the 8 XCDs round-robin by block id: renumber
+  // them so each XCD takes one contiguous range of tiles (neighbouring
+  // tiles, whose halos overlap, then share that XCD's L2), for any grid.
+  {
+    const int64_t x = t & 7, q = nb >> 3, rm = nb & 7;
+    t = x * q + (x < rm ? x : rm) + (t >> 3);
+  }
   if (t >= a.ntiles) return;                               // whole block
   const int lane = threadIdx.x;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
@@ -1798,11 +1804,12 @@ bool fused_pass_supported(int dtype, const Grid3& g, int k) {
 namespace {
 // Device time (us) of one launch at 512^3 on MI355X: index 1 the one-step
 // kernel, 2..8 the fused pass of that many steps (0: no such pass).
-// Rows: FP64, FP32, bf16 round to nearest.
-constexpr float kPassUs[3][kMaxTimeBlock + 1] = {
-    {0, 382, 572, 579, 545, 0, 558, 602, 716},
-    {0, 192, 379, 403, 381, 0, 400, 450, 407},
-    {0, 101, 232, 229, 246, 0, 396, 352, 448}};
+// Rows: FP64, FP32, bf16 round to nearest, bf16 stochastic rounding.
+constexpr float kPassUs[4][kMaxTimeBlock + 1] = {
+    {0, 373, 573, 576, 537, 0, 553, 577, 705},
+    {0, 190, 379, 407, 381, 0, 400, 450, 402},
+    {0, 99, 228, 233, 241, 0, 394, 356, 453},
+    {0, 117, 220, 0, 348, 0, 0, 0, 769}};
 constexpr int kPlanTail = 48;  // steps split exactly; longer runs first take the best depth
 }  // namespace
 
@@ -1812,15 +1819,8 @@ int plan_pass(int dtype, const Grid3& g, int left, int cap) {
   bool ok[kMaxTimeBlock + 1] = {};
   ok[1] = true;
   for (int k = 2; k <= cap; ++k) ok[k] = fused_pass_supported(dtype, g, k);
-  if (dtype == FDB_BF16_SR) {
-    // Stochastic-rounding passes are bound by the hash arithmetic, and the
-    // deepest is the fastest per step (512^3: 110 us at K = 8, 115 at 4,
-    // 153 at 2, 120 one step per launch).  The deepest pass that fits.
-    for (int k = cap; k >= 2; --k)
-      if (ok[k]) return k;
-    return 1;
-  }
-  const float* us = kPassUs[dtype == FDB_F64 ? 0 : (dtype == FDB_F32 ? 1 : 2)];
+  const float* us =
+      kPassUs[dtype == FDB_F64 ? 0 : (dtype == FDB_F32 ? 1 : (dtype == FDB_BF16 ? 2 : 3))];
   for (int k = 2; k <= cap; ++k) ok[k] = ok[k] && us[k] > 0;
   if (left > kPlanTail) {  // the least time per step
     int best = 1;

@@ -390,8 +390,9 @@ def test_auto_plan_covers_every_step_with_fused_depths(dtype, nsteps):
     assert sum(ks) == nsteps
     allowed = {1, 2, 4, 8} if dtype == "BF16_SR" else {1, 2, 3, 4, 6, 7, 8}
     assert set(ks) <= allowed, ks
-    if nsteps >= 8:  # long runs are mostly deep passes
-        assert sum(k for k in ks if k >= 6) >= nsteps - 8, ks
+    if nsteps >= 8:  # long runs are mostly fused passes of each type's best depth
+        deep = 4 if dtype == "BF16_SR" else 6  # (bf16-SR: 4 steps, 87 us per step at 512^3)
+        assert sum(k for k in ks if k >= deep) >= nsteps - 8, ks
 
 
 def test_auto_plan_takes_few_launches():
