@@ -253,15 +253,15 @@ def compare(a_up, b_up):
 
 def mixed_precision(n=1024, steps=200):
     """The Gaussian-bubble workload in FP64, FP32 and BF16 storage (round to
-    nearest even, and stochastic rounding); FP64/FP32 fuse 8 steps per pass
-    (same results)."""
+    nearest even, and stochastic rounding), each split into fused passes by
+    the library's planner (time_block 0; the split changes no value)."""
     from fidibench_amd import configs as K
     w = K.gauss1024(n=n, steps=steps)
     res, ups = {}, {}
     for dt_name in ("f64", "f32", "bf16", "bf16sr"):
         from fidibench_amd import Upwind
         up = Upwind(w.velocity, w.lengths, w.num_cells, dtype=dt_name)
-        up.set_time_block(8)
+        up.set_time_block(0)
         up.init_field(w.init_kind, w.init_params, seed=w.init_seed)
         m0 = up.mass()
         sync()
@@ -435,8 +435,9 @@ def large_adjoint(shape=(1024, 1024, 768), steps=400):
     """The gradient at 1024x1024x768 x 400 steps: BF16-SR forward with FP32
     adjoint by both methods (checkpointed: snapshots sized to the free HBM),
     against the same gradient in FP64; time ratios to the plain BF16
-    forwards over the same steps (round to nearest fused 8 steps per pass,
-    and the stochastic-rounding forward the gradient runs)."""
+    forwards over the same steps (round to nearest, and the stochastic-
+    rounding forward the gradient runs), each split into passes by the
+    library's planner (time_block 0)."""
     import torch
     from fidibench_amd import _lib as L, configs as K
     from fidibench_amd.workloads import advect, compare, init_field, misfit_gradient
@@ -447,8 +448,8 @@ def large_adjoint(shape=(1024, 1024, 768), steps=400):
     g64 = misfit_gradient(w, target, L.F64, L.F64, 64 << 30, method="shift")
     lam64, gv64 = g64.pop("grad_f0"), g64.pop("grad_v")
     fwd = {}
-    for name, code, tb in (("plain_bf16_forward_s", L.BF16, 8),
-                           ("plain_bf16sr_forward_s", L.BF16_SR, 8)):
+    for name, code, tb in (("plain_bf16_forward_s", L.BF16, 0),
+                           ("plain_bf16sr_forward_s", L.BF16_SR, 0)):
         f0 = init_field(w, L.BF16)
         sync()
         gpu_busy()

@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "fdb_internal.h"
 #include "../../include/fidibench_amd/fdb_upwind.h"
 
@@ -1588,9 +1590,14 @@ bool tb_sr_ok(const Grid3& g, int K) {
   return (K == 2 || K == 4 || K == 8) && g.n[1] % 2 == 0;
 }
 
+// visits != nullptr: launch nothing; store the pass's length in plane visits
+// of one resident block slot instead (rounds of resident blocks x planes per
+// block, warm-up planes included), from the same shape, chunking and
+// occupancy the launch would use.
 template <typename TS>
 hipError_t tb_dispatch(const Grid3& g, const TS* in, TS* out, int K, hipStream_t s,
-                       void* const* levels = nullptr, const uint32_t* sr_keys = nullptr) {
+                       void* const* levels = nullptr, const uint32_t* sr_keys = nullptr,
+                       int64_t* visits = nullptr) {
   using M = typename St<TS>::M;
   const bool sr = sr_keys != nullptr;
   if (sr && (sizeof(TS) != 2 || !tb_sr_ok(g, K))) return hipErrorNotSupported;
@@ -1622,17 +1629,22 @@ hipError_t tb_dispatch(const Grid3& g, const TS* in, TS* out, int K, hipStream_t
   if (!launched && R == RR && K == KK && WY == W && lo1 == L1 && lo2 == L2 &&               \
       mode == MD) {                                                                         \
     auto kern = tb_kernel<TS, RR, KK, W, L1, L2, MD>;                                       \
+    const int64_t cap = resident_blocks(kern, 64 * W);                                      \
     int64_t nch;                                                                            \
     if (cfg.blocks > 0) {                                                                   \
       nch = std::max<int64_t>(1, (cfg.blocks + tiles / 2) / tiles);                         \
     } else {                                                                                \
-      nch = march_chunks(tiles, a.n0, KK, resident_blocks(kern, 64 * W), 1, 64, true);      \
+      nch = march_chunks(tiles, a.n0, KK, cap, 1, 64, true);                                \
     }                                                                                       \
     a.chunk = (a.n0 + nch - 1) / nch;                                                       \
     a.ntiles = tiles * ((a.n0 + a.chunk - 1) / a.chunk);                                    \
-    if (a.ntiles > 0x7fffffffLL) return hipErrorInvalidConfiguration;                       \
-    hipLaunchKernelGGL(kern, dim3((unsigned)a.ntiles), dim3(64, W), 0, s, in, out, a, cf,   \
-                       aux);                                                                \
+    if (visits) {                                                                           \
+      *visits = (a.ntiles + cap - 1) / cap * (a.chunk + KK);                                \
+    } else {                                                                                \
+      if (a.ntiles > 0x7fffffffLL) return hipErrorInvalidConfiguration;                     \
+      hipLaunchKernelGGL(kern, dim3((unsigned)a.ntiles), dim3(64, W), 0, s, in, out, a, cf, \
+                         aux);                                                              \
+    }                                                                                       \
     launched = true;                                                                        \
   }
 #define FDB_TB(RR, KK, W, MD)                                                          \
@@ -1659,7 +1671,7 @@ hipError_t tb_dispatch(const Grid3& g, const TS* in, TS* out, int K, hipStream_t
 #undef FDB_TB
 #undef FDB_TB_DIR
   if (!launched) return hipErrorNotSupported;
-  return hipGetLastError();
+  return visits ? hipSuccess : hipGetLastError();
 }
 
 // Fused transpose pass (tb_kernel with PROD): K adjoint steps with the
@@ -1811,17 +1823,115 @@ constexpr float kPassUs[4][kMaxTimeBlock + 1] = {
     {0, 99, 228, 233, 241, 0, 394, 356, 453},
     {0, 117, 220, 0, 348, 0, 0, 0, 769}};
 constexpr int kPlanTail = 48;  // steps split exactly; longer runs first take the best depth
+int pass_row(int dtype) {
+  return dtype == FDB_F64 ? 0 : (dtype == FDB_F32 ? 1 : (dtype == FDB_BF16 ? 2 : 3));
+}
+
+// Length of a k-step fused pass over g in plane visits of one resident block
+// slot (tb_dispatch's query mode); 0 when there is no such pass.
+int64_t fused_pass_visits(int dtype, const Grid3& g, int k) {
+  if (!fused_pass_supported(dtype, g, k)) return 0;
+  int64_t v = 0;
+  hipError_t e = hipErrorNotSupported;
+  const uint32_t keys[kMaxTimeBlock] = {};
+  switch (dtype) {
+    case FDB_F64:
+      e = tb_dispatch<double>(g, nullptr, nullptr, k, nullptr, nullptr, nullptr, &v);
+      break;
+    case FDB_F32:
+      e = tb_dispatch<float>(g, nullptr, nullptr, k, nullptr, nullptr, nullptr, &v);
+      break;
+    case FDB_BF16:
+      e = tb_dispatch<uint16_t>(g, nullptr, nullptr, k, nullptr, nullptr, nullptr, &v);
+      break;
+    case FDB_BF16_SR:
+      e = tb_dispatch<uint16_t>(g, nullptr, nullptr, k, nullptr, nullptr, keys, &v);
+      break;
+    default: break;
+  }
+  return e == hipSuccess ? v : 0;
+}
+
+// Launch times (us) of every depth over g (0: no such pass).  On grids of at
+// least 512^3 cells the 512^3 times are scaled to g: the one-step kernel by
+// the cell count, a fused pass by its length in plane visits (its blocks take
+// equal time per plane there, so the length captures rounds of resident
+// blocks, warm-up planes and tiles cut by the grid's extents; at 1024^3 FP64
+// this picks 8-step passes, 679 us per step, over 7-step ones, 710).  On
+// smaller grids launch times stop scaling (a march of few planes is bound by
+// the latency of each plane, not by throughput), and the 512^3 times are
+// used as they are.
+struct PassCosts {
+  float us[kMaxTimeBlock + 1];
+};
+
+PassCosts pass_costs(int dtype, const Grid3& g) {
+  static const int64_t* ref = [] {  // visits at 512^3, per row and depth
+    static int64_t v[4][kMaxTimeBlock + 1] = {};
+    const uint64_t n[3] = {512, 512, 512};
+    const double vel[3] = {1.0, 1.0, 1.0}, len[3] = {1.0, 1.0, 1.0};
+    Grid3 r;
+    if (make_grid(3, n, vel, len, 1e-4, &r)) {
+      const int dts[4] = {FDB_F64, FDB_F32, FDB_BF16, FDB_BF16_SR};
+      for (int d = 0; d < 4; ++d)
+        for (int k = 2; k <= kMaxTimeBlock; ++k) v[d][k] = fused_pass_visits(dts[d], r, k);
+    }
+    return &v[0][0];
+  }();
+  const int row = pass_row(dtype);
+  const float* tab = kPassUs[row];
+  const double cells = (double)g.ntot / (512.0 * 512.0 * 512.0);
+  const bool scale = cells >= 1.0;
+  PassCosts c{};
+  c.us[1] = scale ? tab[1] * (float)cells : tab[1];
+  for (int k = 2; k <= kMaxTimeBlock; ++k) {
+    const int64_t r = ref[row * (kMaxTimeBlock + 1) + k];
+    if (tab[k] <= 0.0f || !fused_pass_supported(dtype, g, k)) continue;
+    if (!scale) {
+      c.us[k] = tab[k];
+    } else if (r > 0) {
+      const int64_t v = fused_pass_visits(dtype, g, k);
+      if (v > 0) c.us[k] = tab[k] * (float)v / (float)r;
+    }
+  }
+  return c;
+}
+
+// pass_costs for the last few (dtype, grid) pairs: a plan asks once per pass.
+PassCosts cached_pass_costs(int dtype, const Grid3& g) {
+  struct Entry {
+    int dtype;
+    int64_t n[3];
+    int act[3];
+    PassCosts c;
+  };
+  static std::mutex mu;
+  static Entry cache[8];
+  static int used = 0, next = 0;
+  std::lock_guard<std::mutex> lock(mu);
+  for (int i = 0; i < used; ++i) {
+    const Entry& e = cache[i];
+    if (e.dtype == dtype && e.n[0] == g.n[0] && e.n[1] == g.n[1] && e.n[2] == g.n[2] &&
+        e.act[0] == g.act[0] && e.act[1] == g.act[1] && e.act[2] == g.act[2])
+      return e.c;
+  }
+  Entry& e = cache[next];
+  e = Entry{dtype, {g.n[0], g.n[1], g.n[2]}, {g.act[0], g.act[1], g.act[2]},
+            pass_costs(dtype, g)};
+  next = (next + 1) % 8;
+  used = std::min(used + 1, 8);
+  return e.c;
+}
 }  // namespace
 
 int plan_pass(int dtype, const Grid3& g, int left, int cap) {
   cap = std::min(std::min(cap, left), kMaxTimeBlock);
   if (cap <= 1) return 1;
+  const PassCosts pc = cached_pass_costs(dtype, g);
+  const float* us = pc.us;
   bool ok[kMaxTimeBlock + 1] = {};
   ok[1] = true;
-  for (int k = 2; k <= cap; ++k) ok[k] = fused_pass_supported(dtype, g, k);
-  const float* us =
-      kPassUs[dtype == FDB_F64 ? 0 : (dtype == FDB_F32 ? 1 : (dtype == FDB_BF16 ? 2 : 3))];
-  for (int k = 2; k <= cap; ++k) ok[k] = ok[k] && us[k] > 0;
+  for (int k = 2; k <= cap; ++k) ok[k] = us[k] > 0;
   if (left > kPlanTail) {  // the least time per step
     int best = 1;
     for (int k = 2; k <= cap; ++k)

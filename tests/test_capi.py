@@ -382,6 +382,22 @@ def test_auto_plan_of_twenty_fp64_steps_is_seven_seven_six():
     assert sorted(_passes(L.F64, 20, 0), reverse=True) == [7, 7, 6]
 
 
+def test_auto_plan_scales_launch_times_to_larger_grids():
+    """From 512^3 cells up the planner scales each depth's launch time by the
+    pass's length on the grid (rounds of resident blocks x planes marched
+    per block): at 1024^3 FP64 runs mostly 8-step passes (679 us per step
+    measured on MI355X against 710 for 7-step ones), at 512^3 7-step passes
+    (86 us per step against 91).  Smaller grids keep the 512^3 times."""
+    L = _lib()
+    big = ((1024, 1024, 1024), (0.7, -0.4, 0.2), (1.0, 1.0, 1.0))
+    ks = _passes(L.F64, 100, 0, big)
+    assert sum(ks) == 100 and ks.count(8) >= 9, ks
+    ks = _passes(L.F64, 100, 0)
+    assert ks.count(7) >= 10, ks
+    small = ((21, 70, 130), (0.3, -0.45, 0.2), (1.0, 1.0, 1.0))
+    assert sorted(_passes(L.F64, 20, 0, small), reverse=True) == [7, 7, 6]
+
+
 @pytest.mark.parametrize("dtype", ["F64", "F32", "BF16", "BF16_SR"])
 @pytest.mark.parametrize("nsteps", [1, 2, 5, 9, 13, 20, 47, 48, 49, 100, 400])
 def test_auto_plan_covers_every_step_with_fused_depths(dtype, nsteps):
