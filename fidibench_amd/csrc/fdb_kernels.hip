@@ -1865,8 +1865,10 @@ struct PassCosts {
   float us[kMaxTimeBlock + 1];
 };
 
-PassCosts pass_costs(int dtype, const Grid3& g) {
-  static const int64_t* ref = [] {  // visits at 512^3, per row and depth
+// Plane visits of every pass at 512^3, per row of kPassUs and depth (read
+// once per process, on the first grid that needs them).
+const int64_t* ref_visits() {
+  static const int64_t* ref = [] {
     static int64_t v[4][kMaxTimeBlock + 1] = {};
     const uint64_t n[3] = {512, 512, 512};
     const double vel[3] = {1.0, 1.0, 1.0}, len[3] = {1.0, 1.0, 1.0};
@@ -1878,20 +1880,24 @@ PassCosts pass_costs(int dtype, const Grid3& g) {
     }
     return &v[0][0];
   }();
+  return ref;
+}
+
+PassCosts pass_costs(int dtype, const Grid3& g) {
   const int row = pass_row(dtype);
   const float* tab = kPassUs[row];
   const double cells = (double)g.ntot / (512.0 * 512.0 * 512.0);
   const bool scale = cells >= 1.0;
+  const int64_t* ref = scale ? ref_visits() + row * (kMaxTimeBlock + 1) : nullptr;
   PassCosts c{};
   c.us[1] = scale ? tab[1] * (float)cells : tab[1];
   for (int k = 2; k <= kMaxTimeBlock; ++k) {
-    const int64_t r = ref[row * (kMaxTimeBlock + 1) + k];
     if (tab[k] <= 0.0f || !fused_pass_supported(dtype, g, k)) continue;
     if (!scale) {
       c.us[k] = tab[k];
-    } else if (r > 0) {
+    } else if (ref[k] > 0) {
       const int64_t v = fused_pass_visits(dtype, g, k);
-      if (v > 0) c.us[k] = tab[k] * (float)v / (float)r;
+      if (v > 0) c.us[k] = tab[k] * (float)v / (float)ref[k];
     }
   }
   return c;

@@ -128,7 +128,7 @@ class Upwind:
 
     def plan_passes(self, numTimeSteps: int, deltaTime: float, time_block: int) -> list[int]:
         """Steps per launch of advect(numTimeSteps, deltaTime) with this
-        time_block (no device work)."""
+        time_block (launches nothing)."""
         return L.plan_passes(self.numCells, self.velocity, self.lengths, deltaTime, self.dtype,
                              numTimeSteps, time_block)
 

@@ -163,7 +163,9 @@ int fdb_advect_dev(int ndims, const uint64_t* num_cells, const double* velocity,
  * nsteps steps with this time_block (0..8, 0 = FDB_TIME_BLOCK_AUTO) makes
  * for this geometry and dtype, as fdb_upwind_advect and fdb_advect_dev
  * run them: *count gets the number of launches, ks[0 .. min(count,
- * max_ks)-1] their depths.  Host-only (no device work).  Extension (no
+ * max_ks)-1] their depths.  Launches and allocates nothing; on grids of at
+ * least 512^3 cells the split reads the kernels' occupancy on the current
+ * device (with no device: one block per CU on 256 CUs).  Extension (no
  * counterpart in the reference). */
 int fdb_plan_passes(int ndims, const uint64_t* num_cells, const double* velocity,
                     const double* lengths, double delta_time, int dtype, int nsteps,
