@@ -298,6 +298,7 @@ def forward_adjoint(w, steps, f_dtype, fwd_ms, method):
                                        f0.data_ptr(), torch.cuda.current_stream().cuda_stream))
     fT = torch.empty_like(f0)
     lam0 = torch.empty_like(f0)
+    lamT = torch.empty_like(f0)
     tape = Tape(w.num_cells, w.velocity, w.lengths, w.dt, steps, fd, fd, default_budget(0.8),
                 method=method)
     e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
@@ -306,7 +307,7 @@ def forward_adjoint(w, steps, f_dtype, fwd_ms, method):
     e0.record()
     tape.forward(f0, fT)
     e1.record()
-    lamT = fT.clone()  # dL/df_T of 0.5*|f_T|^2, a stand-in misfit gradient
+    lamT.copy_(fT)  # dL/df_T of 0.5*|f_T|^2, a stand-in misfit gradient
     gv = tape.backward(lamT, lam0)
     e2.record()
     sync()

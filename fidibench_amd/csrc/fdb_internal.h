@@ -108,6 +108,17 @@ hipError_t launch_adjoint_fused(int lam_dtype, int f_dtype, const Grid3& g, cons
                                 int64_t max_blocks, double* acc, hipStream_t s);
 int64_t adjoint_fused_blocks(int lam_dtype, int f_dtype, const Grid3& g, int k);
 
+// K transpose steps in one pass whose last step also adds its velocity
+// products against f (the forward state that step consumes) into acc:
+// K - 1 plain transpose steps and one launch_adjoint_step with f_t = f,
+// lam bit-identical, the products summed in another order.  k must be
+// adjoint_last_depth(...), which returns 0 when the pairing or shape has
+// no such pass and sets *nb to the partial slots it needs.
+hipError_t launch_adjoint_last(int lam_dtype, int f_dtype, const Grid3& g, const void* lam_next,
+                               void* lam, const void* f, int k, double* partials,
+                               int64_t max_blocks, double* acc, hipStream_t s);
+int adjoint_last_depth(int lam_dtype, int f_dtype, const Grid3& g, int64_t* nb);
+
 // Launch sizing shared by the marching kernels.
 int device_cus();
 // Blocks of kernel `kern` (host stub) with `threads` threads and `dyn_lds`

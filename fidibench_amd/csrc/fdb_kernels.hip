@@ -834,6 +834,10 @@ struct TbArgs {
 //                the one-step adjoint's velocity products -- in FP64
 //                (differences taken in FP64 as there), Neumaier-compensated
 //                per thread and reduced per block into aux.partials[block*6].
+//   kTbAdjLast   the fused transpose pass with the products of its last
+//                level only, against aux.fs[0] (the forward state that
+//                level's step consumes): K - 1 plain transpose steps, then
+//                the one-step adjoint with products.
 //   kTbSR        K forward steps in bf16 storage, each level rounded
 //                stochastically with the bits a one-step launch of that step
 //                draws (sr_pair_hash of the cell's pair and the step's key,
@@ -842,7 +846,7 @@ struct TbArgs {
 //                lane holds both rows of each of its pairs (even rows per
 //                wave, even K, even n1), so each lane hashes one word per
 //                pair of rows and level, with no exchange between lanes.
-constexpr int kTbPlain = 0, kTbStoreAll = 1, kTbAdjoint = 2, kTbSR = 3;
+constexpr int kTbPlain = 0, kTbStoreAll = 1, kTbAdjoint = 2, kTbSR = 3, kTbAdjLast = 4;
 struct TbAux {
   const void* fs[kMaxTimeBlock];
   void* lv[kMaxTimeBlock];
@@ -889,7 +893,11 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
                                                      TS* __restrict__ out, TbArgs a,
                                                      Coef<typename St<TS>::M> cf, TbAux adj) {
   using M = typename St<TS>::M;
-  constexpr bool PROD = MODE == kTbAdjoint, ALL = MODE == kTbStoreAll, SR = MODE == kTbSR;
+  constexpr bool ALL = MODE == kTbStoreAll, SR = MODE == kTbSR, LASTP = MODE == kTbAdjLast;
+  // PROD: velocity products (at every level, or at the last one only);
+  // NF states held, one per level with products.
+  constexpr bool PROD = MODE == kTbAdjoint || LASTP;
+  constexpr int NF = MODE == kTbAdjoint ? K : 1;
   static_assert(!SR || sizeof(TS) == 2, "stochastic rounding is for bf16 storage");
   // FP64 forward passes exchange the edge rows one level ahead (PIPE; see
   // the level loop): their 200-240 VGPRs leave one 8-wave block per CU, so
@@ -977,13 +985,13 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
   constexpr bool kTwoPlanes = !PROD && (K <= 6 || (sizeof(M) == 8 && R * K == 56));
   constexpr bool kCondPrefetch = PIPE && !kTwoPlanes;
   constexpr bool kWaitBeforeStores = (MODE == kTbPlain || MODE == kTbSR) && !kCondPrefetch;
-  // PROD: fv[l] holds the state of level l at the current plane; after
-  // level l it is reloaded with the next plane's (always a valid plane, so
-  // every load is unconditional).  The states are needed at output cells
-  // only: halo lanes and rows load the nearest output cell of the tile
-  // instead (a line the tile loads anyway), so the K states cost no halo
-  // traffic.
-  M fv[PROD ? K : 1][R];
+  // PROD: fv[l] holds the state of level l at the current plane (LASTP:
+  // fv[0], the last level's); after its level it is reloaded with the next
+  // plane's (always a valid plane, so every load is unconditional).  The
+  // states are needed at output cells only: halo lanes and rows load the
+  // nearest output cell of the tile instead (a line the tile loads
+  // anyway), so the states cost no halo traffic.
+  M fv[NF][R];
   VAcc acc;
   uint32_t colf = col;
   int64_t rofff[PROD ? R : 1];
@@ -997,7 +1005,7 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
       rofff[r] = (rl < o1 ? o1 : (rl > r_hi ? r_hi : rl)) * a.n2;
     }
 #pragma unroll
-    for (int l = 0; l < K; ++l) {
+    for (int l = 0; l < NF; ++l) {
       const TF* fb = (const TF*)adj.fs[l] + pw * plane;
 #pragma unroll
       for (int r = 0; r < R; ++r) fv[l][r] = (M)St<TF>::ld((fb + rofff[r])[colf]);
@@ -1118,11 +1126,12 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
         if (LO1 ? w > 0 : w + 1 < WY) n1 = e;
       }
       d1[er] = n1 - x[er];
-      if constexpr (PROD) {
+      if constexpr (PROD) if (!LASTP || l + 1 == K) {
+        const int fl = LASTP ? 0 : l;
         const bool cells = m >= K && col_out;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          const double fd = (double)fv[l][r];
+          const double fd = (double)fv[fl][r];
           double q0, q1, q2;
           if constexpr (sizeof(M) == 8) {  // the FP64 differences are d0..d2
             q0 = fd * d0[r];
@@ -1140,9 +1149,9 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
           ps[1] += ok ? q1 : 0.0;
           ps[2] += ok ? q2 : 0.0;
         }
-        const TF* fb = (const TF*)adj.fs[l] + pn * plane;
+        const TF* fb = (const TF*)adj.fs[fl] + pn * plane;
 #pragma unroll
-        for (int r = 0; r < R; ++r) fv[l][r] = (M)St<TF>::ld((fb + rofff[r])[colf]);
+        for (int r = 0; r < R; ++r) fv[fl][r] = (M)St<TF>::ld((fb + rofff[r])[colf]);
       }
 #pragma unroll
       for (int r = 0; r < R; ++r) {
@@ -1677,17 +1686,22 @@ hipError_t tb_dispatch(const Grid3& g, const TS* in, TS* out, int K, hipStream_t
 // Fused transpose pass (tb_kernel with PROD): K adjoint steps with the
 // velocity products, K = 4, 8 waves of 4 rows (FP64) or 8 rows (FP32):
 // prev, the K states and the level's differences then stay within 256
-// VGPRs without spills (FP64 spills from 5 rows on).  *nb_out gets
-// the block count (= partial slots written); with dry = true nothing is
-// launched.  hipErrorNotSupported when the shape has no fused form or
+// VGPRs without spills (FP64 spills from 5 rows on).  LAST: the products
+// of the last step only, against fs[0] (kTbAdjLast): K = 7 with 8 rows
+// (FP64, 234 VGPRs, one block per CU like the plain 7-step pass) or K = 8
+// with 6 rows (FP32, at most 115 VGPRs, so two blocks per CU).  *nb_out
+// gets the block count (= partial slots written); with dry = true nothing
+// is launched.  hipErrorNotSupported when the shape has no fused form or
 // needs more than max_blocks partial slots.
-template <typename TS, typename TF = TS>
+template <typename TS, typename TF = TS, bool LAST = false>
 hipError_t tb_adj_dispatch(const Grid3& g, const TS* in, TS* out, const void* const* fs, int K,
                            double* partials, int64_t max_blocks, int64_t* nb_out, bool dry,
                            hipStream_t s) {
   using M = typename St<TS>::M;
   if (sizeof(TS) < 4 || K > kMaxTimeBlock) return hipErrorNotSupported;
-  constexpr int R = sizeof(TS) == 8 ? 4 : 8, WY = 8;
+  constexpr int R = LAST ? (sizeof(TS) == 8 ? 8 : 6) : (sizeof(TS) == 8 ? 4 : 8), WY = 8;
+  constexpr int KL = LAST ? (sizeof(TS) == 8 ? 7 : 8) : 4;  // the instantiated depth
+  constexpr int MD = LAST ? kTbAdjLast : kTbAdjoint;
   if (!(g.act[0] && g.act[1] && g.act[2]) || WY * R <= K) return hipErrorNotSupported;
   TbArgs a;
   a.n0 = g.n[0];
@@ -1702,12 +1716,12 @@ hipError_t tb_adj_dispatch(const Grid3& g, const TS* in, TS* out, const void* co
   Coef<M> cf;
   for (int j = 0; j < 3; ++j) cf.c[j] = (M)g.c[j];
   TbAux adj{};
-  for (int l = 0; l < K; ++l) adj.fs[l] = fs ? fs[l] : nullptr;
+  for (int l = 0; l < (LAST ? 1 : K); ++l) adj.fs[l] = fs ? fs[l] : nullptr;
   adj.partials = partials;
   bool launched = false;
 #define FDB_TBA_DIR(KK, L1, L2)                                                              \
   if (!launched && K == KK && lo1 == L1 && lo2 == L2) {                                      \
-    auto kern = tb_kernel<TS, R, KK, WY, L1, L2, kTbAdjoint, TF>;                            \
+    auto kern = tb_kernel<TS, R, KK, WY, L1, L2, MD, TF>;                                    \
     const int64_t nch =                                                                      \
         march_chunks(tiles, a.n0, KK, resident_blocks(kern, 64 * WY), 1, 64, true);          \
     a.chunk = (a.n0 + nch - 1) / nch;                                                        \
@@ -1722,7 +1736,7 @@ hipError_t tb_adj_dispatch(const Grid3& g, const TS* in, TS* out, const void* co
 #define FDB_TBA(KK)                                                  \
   FDB_TBA_DIR(KK, false, false) FDB_TBA_DIR(KK, false, true)         \
   FDB_TBA_DIR(KK, true, false) FDB_TBA_DIR(KK, true, true)
-  FDB_TBA(4)
+  FDB_TBA(KL)
 #undef FDB_TBA
 #undef FDB_TBA_DIR
   if (!launched) return hipErrorNotSupported;
@@ -2061,31 +2075,57 @@ namespace {
 
 // The (lam, f_t) storage pairings with a fused transpose pass: FP64/FP64,
 // FP32/FP32 and FP32 lam over bf16 states (the mixed-precision gradient).
+template <bool LAST>
 hipError_t adj_fused_typed(int lam_dtype, int f_dtype, const Grid3& g, const void* lam_next,
                            void* lam, const void* const* fs, int k, double* partials,
                            int64_t max_blocks, int64_t* nb, bool dry, hipStream_t s) {
   f_dtype = storage_dtype(f_dtype);
   if (lam_dtype == FDB_F64 && f_dtype == FDB_F64)
-    return tb_adj_dispatch<double>(g, (const double*)lam_next, (double*)lam, fs, k, partials,
-                                   max_blocks, nb, dry, s);
+    return tb_adj_dispatch<double, double, LAST>(g, (const double*)lam_next, (double*)lam, fs,
+                                                 k, partials, max_blocks, nb, dry, s);
   if (lam_dtype == FDB_F32 && f_dtype == FDB_F32)
-    return tb_adj_dispatch<float>(g, (const float*)lam_next, (float*)lam, fs, k, partials,
-                                  max_blocks, nb, dry, s);
+    return tb_adj_dispatch<float, float, LAST>(g, (const float*)lam_next, (float*)lam, fs, k,
+                                               partials, max_blocks, nb, dry, s);
   if (lam_dtype == FDB_F32 && f_dtype == FDB_BF16)
-    return tb_adj_dispatch<float, uint16_t>(g, (const float*)lam_next, (float*)lam, fs, k,
-                                            partials, max_blocks, nb, dry, s);
+    return tb_adj_dispatch<float, uint16_t, LAST>(g, (const float*)lam_next, (float*)lam, fs,
+                                                  k, partials, max_blocks, nb, dry, s);
   return hipErrorNotSupported;
 }
 
 }  // namespace
+
+hipError_t launch_adjoint_last(int lam_dtype, int f_dtype, const Grid3& g, const void* lam_next,
+                               void* lam, const void* f, int k, double* partials,
+                               int64_t max_blocks, double* acc, hipStream_t s) {
+  if (lam_next == lam || f == nullptr || k < 2) return hipErrorNotSupported;
+  int64_t nb = 0;
+  const void* fs[1] = {f};
+  const hipError_t err = adj_fused_typed<true>(lam_dtype, f_dtype, g, lam_next, lam, fs, k,
+                                               partials, max_blocks, &nb, false, s);
+  if (err != hipSuccess) return err;
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, partials, nb, 3, acc, 1);
+  return hipGetLastError();
+}
+
+int adjoint_last_depth(int lam_dtype, int f_dtype, const Grid3& g, int64_t* nb) {
+  for (int k : {7, 8}) {
+    int64_t n = 0;
+    if (adj_fused_typed<true>(lam_dtype, f_dtype, g, nullptr, nullptr, nullptr, k, nullptr, 0,
+                              &n, true, nullptr) == hipSuccess) {
+      if (nb) *nb = n;
+      return k;
+    }
+  }
+  return 0;
+}
 
 hipError_t launch_adjoint_fused(int lam_dtype, int f_dtype, const Grid3& g, const void* lam_next,
                                 void* lam, const void* const* fs, int k, double* partials,
                                 int64_t max_blocks, double* acc, hipStream_t s) {
   if (lam_next == lam || k < 2) return hipErrorNotSupported;
   int64_t nb = 0;
-  const hipError_t err = adj_fused_typed(lam_dtype, f_dtype, g, lam_next, lam, fs, k, partials,
-                                         max_blocks, &nb, false, s);
+  const hipError_t err = adj_fused_typed<false>(lam_dtype, f_dtype, g, lam_next, lam, fs, k,
+                                                partials, max_blocks, &nb, false, s);
   if (err != hipSuccess) return err;
   hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, partials, nb, 3, acc, 1);
   return hipGetLastError();
@@ -2093,8 +2133,8 @@ hipError_t launch_adjoint_fused(int lam_dtype, int f_dtype, const Grid3& g, cons
 
 int64_t adjoint_fused_blocks(int lam_dtype, int f_dtype, const Grid3& g, int k) {
   int64_t nb = 0;
-  (void)adj_fused_typed(lam_dtype, f_dtype, g, nullptr, nullptr, nullptr, k, nullptr, 0, &nb,
-                        true, nullptr);
+  (void)adj_fused_typed<false>(lam_dtype, f_dtype, g, nullptr, nullptr, nullptr, k, nullptr, 0,
+                               &nb, true, nullptr);
   return nb;
 }
 

@@ -29,7 +29,8 @@
 // The backward is then T-1 plain transpose steps (fused K-step passes of
 // the forward kernel with the directions negated; lam_0 bit-identical to
 // the checkpointed sweep, whose lam recursion does not involve f either)
-// and one transpose step with the products against f_0: no state but f_0
+// and one transpose step with the products against f_0 (the last level of
+// the last fused pass where the storage pairing allows): no state but f_0
 // is needed, so nothing is stored or recomputed.
 #include <hip/hip_runtime_api.h>
 
@@ -400,16 +401,22 @@ int ensure_partials(fdb_tape* t, int64_t nb) {
 // (fused passes of the forward kernel run with the directions negated,
 // same coefficients), then lam_1 -> lam_0 with the velocity products
 // against f_0, scaled by T (the products of every step are equal; see the
-// file comment).  lam_T is only read; the passes alternate between lam_tmp
-// and lam0, arranged so that lam_1 lands in lam_tmp and the last step
-// writes lam0.
+// file comment).  Where the storage pairing has one, the last L steps
+// (lam_L -> lam_0) run as one fused pass that forms the products at its
+// last level (launch_adjoint_last: FP64 L = 7, FP32 L = 8), which saves
+// the one-step launch's pass over lam_1, f_0 and lam_0.  lam_T is only
+// read; the passes alternate between lam_tmp and lam0, arranged so that
+// the last one writes lam0.
 int backward_shift(fdb_tape* t, const void* lam_T, void* lam0, double* grad_v, hipStream_t s) {
   fdb::Grid3 ga = t->g;  // the transpose step: gather from the downwind side
   for (int j = 0; j < 3; ++j) ga.up[j] = -t->g.up[j];
   const int ld = t->lam_dtype;
-  // Pass lengths for the T-1 plain steps: the deepest fused depth that fits.
+  int64_t nbl = 0;
+  const int kl = fdb::adjoint_last_depth(ld, t->f_dtype, t->g, &nbl);
+  const int last = (kl > 1 && kl <= t->time_block && kl <= t->nsteps) ? kl : 1;
+  // Pass lengths for the T-last plain steps: the planner's split.
   std::vector<int> ks;
-  for (int left = t->nsteps - 1; left > 0;) {
+  for (int left = t->nsteps - last; left > 0;) {
     const int k = pass_steps(ld, ga, left, t->time_block);
     ks.push_back(k);
     left -= k;
@@ -426,12 +433,19 @@ int backward_shift(fdb_tape* t, const void* lam_T, void* lam0, double* grad_v, h
     src = dst;
     t->adj_steps += ks[i];
   }
-  const int64_t nb = fdb::adjoint_blocks(ld, t->f_dtype, t->g, src, lam0, t->f0_ref);
-  if (int rc = ensure_partials(t, nb)) return rc;
-  TAPE_HIP(fdb::launch_zero(t->acc, 6, s));
-  TAPE_HIP(fdb::launch_adjoint_step(ld, t->f_dtype, t->g, src, lam0, t->f0_ref, t->partials,
-                                    t->acc, s));
-  t->adj_steps += 1;
+  if (last > 1) {
+    if (int rc = ensure_partials(t, nbl)) return rc;
+    TAPE_HIP(fdb::launch_zero(t->acc, 6, s));
+    TAPE_HIP(fdb::launch_adjoint_last(ld, t->f_dtype, t->g, src, lam0, t->f0_ref, last,
+                                      t->partials, nbl, t->acc, s));
+  } else {
+    const int64_t nb = fdb::adjoint_blocks(ld, t->f_dtype, t->g, src, lam0, t->f0_ref);
+    if (int rc = ensure_partials(t, nb)) return rc;
+    TAPE_HIP(fdb::launch_zero(t->acc, 6, s));
+    TAPE_HIP(fdb::launch_adjoint_step(ld, t->f_dtype, t->g, src, lam0, t->f0_ref, t->partials,
+                                      t->acc, s));
+  }
+  t->adj_steps += last;
   double acc[6];
   TAPE_HIP(hipMemcpyAsync(acc, t->acc, sizeof(acc), hipMemcpyDeviceToHost, s));
   TAPE_HIP(hipStreamSynchronize(s));

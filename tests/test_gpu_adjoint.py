@@ -231,6 +231,44 @@ def test_shift_method_equals_checkpointed_sweep(dtype, shape, v, T):
     assert np.allclose(gs, gc, rtol=1e-11 if dtype == "f64" else 2e-5, atol=0), (gs, gc)
 
 
+@pytest.mark.parametrize("pair", ["f64/f64", "f32/f32", "bf16/f32"])
+@pytest.mark.parametrize("shape,v,T", [((16, 66, 130), (0.45, -0.3, 0.2), 7),
+                                       ((21, 50, 67), (-0.6, 0.25, -0.3), 8),
+                                       ((9, 130, 70), (0.3, 0.4, -0.2), 23)])
+def test_shift_last_pass_products_equal_one_step_products(pair, shape, v, T):
+    """The shift-invariant backward ends with one fused pass that forms the
+    velocity products at its last level (7 steps for FP64 lam, 8 for FP32
+    lam, over f_0 in the forward's storage); with time_block 1 every step
+    is its own launch and the products come from the one-step adjoint.
+    lam_0 is the same bit for bit, dL/dv the same products summed in
+    another order; T = 7 with FP32 lam (fewer steps than the pass) takes
+    the one-step form either way."""
+    from fidibench_amd import _lib as L
+    from fidibench_amd.autograd import Tape
+    names = {"f64": (L.F64, torch.float64), "f32": (L.F32, torch.float32),
+             "bf16": (L.BF16, torch.bfloat16)}
+    (fd, ft), (ld, lt) = (names[x] for x in pair.split("/"))
+    g = torch.Generator().manual_seed(T + len(pair))
+    f0 = torch.rand(shape, generator=g, dtype=torch.float64).to(ft).cuda()
+    lamT = torch.randn(shape, generator=g, dtype=torch.float64).to(lt).cuda()
+    dt = 0.5 / max(shape) / sum(abs(x) for x in v)
+    out = []
+    for tb in (8, 1):
+        tape = Tape(shape, v, (1, 1, 1), dt, T, fd, ld, 1 << 31, method="shift")
+        tape.set_time_block(tb)
+        fT = torch.empty_like(f0)
+        tape.forward(f0, fT)
+        lam0 = torch.empty_like(lamT)
+        gv = np.array(tape.backward(lamT, lam0))
+        assert tape.stats()["adjoint_steps"] == T
+        tape.close()
+        out.append((fT.cpu(), lam0.cpu(), gv))
+    (fa, la, ga), (fb, lb, gb) = out
+    assert torch.equal(fa.view(torch.uint8), fb.view(torch.uint8))
+    assert torch.equal(la.view(torch.uint8), lb.view(torch.uint8)), (pair, shape, T)
+    assert np.all(np.isfinite(ga)) and np.allclose(ga, gb, rtol=1e-12, atol=0), (ga, gb)
+
+
 @pytest.mark.parametrize("snaps", [0, 1, 3])
 def test_checkpointing_is_exact(snaps):
     """Few snapshots (recomputation) gives the same dL/df0 bits as storing
