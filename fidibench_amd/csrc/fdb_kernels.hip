@@ -905,7 +905,10 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
   // second block.  FP32 and bf16 blocks fit two per CU, which hides them,
   // and there the ordering below (exchange within the level) measured
   // faster (FP32 K=8 2480 vs 2160 G cell-updates/s at 512^3).
-  constexpr bool PIPE = !PROD && sizeof(M) == 8;
+  // The last-level products (LASTP) fit this ordering too: row er's axis-1
+  // neighbour is read at the start of the level, so every row's three
+  // differences are at hand when it is computed.
+  constexpr bool PIPE = MODE != kTbAdjoint && sizeof(M) == 8;
   // Edge row handed to the neighbouring wave, double-buffered by the
   // parity of the running level count (one barrier per level suffices).
   __shared__ M edge[2][WY][64];
@@ -1075,6 +1078,16 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
           const M n2 = LO2 ? dpp_shift_any<0x138>(x[r]) : dpp_shift_any<0x130>(x[r]);
           const M d0 = prev[l][r] - x[r];
           const M d2 = n2 - x[r];
+          if constexpr (LASTP) {
+            if (l + 1 == K) {  // the products of the pass's last step, at output cells
+              const M v1 = r == er ? n1 : x[LO1 ? (r > 0 ? r - 1 : 0) : (r + 1 < R ? r + 1 : R - 1)];
+              const double fd = (double)fv[0][r];
+              const bool ok = m >= K && col_out && row_out[r];
+              ps[0] += ok ? fd * d0 : 0.0;
+              ps[1] += ok ? fd * (v1 - x[r]) : 0.0;
+              ps[2] += ok ? fd * d2 : 0.0;
+            }
+          }
           const M p0 = cf.c[0] * d0;
           const M p2 = cf.c[2] * d2;
           g[r] = x[r] - p0;
@@ -1098,6 +1111,13 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
         for (int r = 0; r < R; ++r) {
           prev[l][r] = x[r];
           x[r] = St<TS>::rnd(g[r]);  // each level as stored
+        }
+        if constexpr (LASTP) {
+          if (l + 1 == K) {  // f_0 at the next plane's output cells
+            const TF* fb = (const TF*)adj.fs[0] + pn * plane;
+#pragma unroll
+            for (int r = 0; r < R; ++r) fv[0][r] = (M)St<TF>::ld((fb + rofff[r])[colf]);
+          }
         }
         if constexpr (WY > 1) {
           // The next plane's row wr (its level 0) as late as possible, so
