@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
 #include <mutex>
 
 #include "fdb_internal.h"
@@ -1848,14 +1849,33 @@ bool fused_pass_supported(int dtype, const Grid3& g, int k) {
 }
 
 namespace {
-// Device time (us) of one launch at 512^3 on MI355X: index 1 the one-step
-// kernel, 2..8 the fused pass of that many steps (0: no such pass).
-// Rows: FP64, FP32, bf16 round to nearest, bf16 stochastic rounding.
-constexpr float kPassUs[4][kMaxTimeBlock + 1] = {
-    {0, 373, 573, 576, 537, 0, 553, 577, 705},
-    {0, 190, 379, 407, 381, 0, 400, 450, 402},
-    {0, 99, 228, 233, 241, 0, 394, 356, 453},
-    {0, 117, 220, 0, 348, 0, 0, 0, 769}};
+// Device time (us) of one launch on MI355X, on cubic grids of kPassSide[i]^3
+// cells: index 1 the one-step kernel, 2..8 the fused pass of that many
+// steps (0: no such pass).  Rows: FP64, FP32, bf16 round to nearest, bf16
+// stochastic rounding.  Below 512^3 a launch's time is bound by the
+// latency of its few marched planes rather than by throughput (at 16^3 a
+// one-step FP64 launch, 16 planes in sequence, takes longer than a 4-step
+// pass), so the times are measured points, not a model.
+constexpr int kPassSizes = 4;
+constexpr double kPassSide[kPassSizes] = {16, 128, 256, 512};
+constexpr float kPassUsAt[kPassSizes][4][kMaxTimeBlock + 1] = {
+    {{0, 23.4f, 7.9f, 8.8f, 9.9f, 0, 20.5f, 23.4f, 32.3f},
+     {0, 12.1f, 6.8f, 8.8f, 10.2f, 0, 17.9f, 23.8f, 26.5f},
+     {0, 11.0f, 9.3f, 8.5f, 9.9f, 0, 20.9f, 25.6f, 28.3f},
+     {0, 10.8f, 6.7f, 0, 10.5f, 0, 0, 0, 37.7f}},
+    {{0, 11.2f, 11.3f, 14.7f, 20.5f, 0, 30.2f, 35.2f, 44.4f},
+     {0, 6.0f, 10.5f, 13.0f, 16.2f, 0, 29.0f, 37.5f, 44.8f},
+     {0, 6.1f, 10.8f, 13.2f, 17.6f, 0, 28.0f, 41.1f, 49.5f},
+     {0, 5.9f, 13.5f, 0, 24.8f, 0, 0, 0, 76.4f}},
+    {{0, 62.8f, 65.0f, 66.9f, 67.9f, 0, 88.3f, 100.4f, 114.5f},
+     {0, 30.4f, 39.2f, 41.5f, 43.4f, 0, 57.0f, 70.8f, 80.2f},
+     {0, 17.7f, 28.9f, 35.8f, 44.6f, 0, 72.5f, 78.7f, 91.7f},
+     {0, 20.7f, 33.9f, 0, 67.7f, 0, 0, 0, 144.0f}},
+    {{0, 373, 573, 576, 537, 0, 553, 577, 705},
+     {0, 190, 379, 407, 381, 0, 400, 450, 402},
+     {0, 99, 228, 233, 241, 0, 394, 356, 453},
+     {0, 117, 220, 0, 348, 0, 0, 0, 769}}};
+constexpr const float (&kPassUs)[4][kMaxTimeBlock + 1] = kPassUsAt[kPassSizes - 1];
 constexpr int kPlanTail = 48;  // steps split exactly; longer runs first take the best depth
 int pass_row(int dtype) {
   return dtype == FDB_F64 ? 0 : (dtype == FDB_F32 ? 1 : (dtype == FDB_BF16 ? 2 : 3));
@@ -1892,9 +1912,10 @@ int64_t fused_pass_visits(int dtype, const Grid3& g, int k) {
 // equal time per plane there, so the length captures rounds of resident
 // blocks, warm-up planes and tiles cut by the grid's extents; at 1024^3 FP64
 // this picks 8-step passes, 679 us per step, over 7-step ones, 710).  On
-// smaller grids launch times stop scaling (a march of few planes is bound by
-// the latency of each plane, not by throughput), and the 512^3 times are
-// used as they are.
+// smaller grids launch times stop scaling that way (a march of few planes is
+// bound by the latency of each plane, not by throughput), and each time is
+// interpolated between the measured sizes, linearly in log(time) against
+// log(cells) (grids below 16^3 take the 16^3 times).
 struct PassCosts {
   float us[kMaxTimeBlock + 1];
 };
@@ -1921,18 +1942,27 @@ PassCosts pass_costs(int dtype, const Grid3& g) {
   const int row = pass_row(dtype);
   const float* tab = kPassUs[row];
   const double cells = (double)g.ntot / (512.0 * 512.0 * 512.0);
-  const bool scale = cells >= 1.0;
-  const int64_t* ref = scale ? ref_visits() + row * (kMaxTimeBlock + 1) : nullptr;
   PassCosts c{};
-  c.us[1] = scale ? tab[1] * (float)cells : tab[1];
-  for (int k = 2; k <= kMaxTimeBlock; ++k) {
-    if (tab[k] <= 0.0f || !fused_pass_supported(dtype, g, k)) continue;
-    if (!scale) {
-      c.us[k] = tab[k];
-    } else if (ref[k] > 0) {
-      const int64_t v = fused_pass_visits(dtype, g, k);
-      if (v > 0) c.us[k] = tab[k] * (float)v / (float)ref[k];
+  if (cells < 1.0) {
+    // Between the measured sizes i and i + 1 (side^3 cells each).
+    const double lc = std::log((double)g.ntot);
+    int i = 0;
+    while (i + 2 < kPassSizes && lc > 3.0 * std::log(kPassSide[i + 1])) ++i;
+    const double l0 = 3.0 * std::log(kPassSide[i]), l1 = 3.0 * std::log(kPassSide[i + 1]);
+    const double a = std::min(1.0, std::max(0.0, (lc - l0) / (l1 - l0)));
+    for (int k = 1; k <= kMaxTimeBlock; ++k) {
+      const float t0 = kPassUsAt[i][row][k], t1 = kPassUsAt[i + 1][row][k];
+      if (t0 <= 0.0f || t1 <= 0.0f || (k > 1 && !fused_pass_supported(dtype, g, k))) continue;
+      c.us[k] = (float)std::exp((1.0 - a) * std::log((double)t0) + a * std::log((double)t1));
     }
+    return c;
+  }
+  const int64_t* ref = ref_visits() + row * (kMaxTimeBlock + 1);
+  c.us[1] = tab[1] * (float)cells;
+  for (int k = 2; k <= kMaxTimeBlock; ++k) {
+    if (tab[k] <= 0.0f || ref[k] <= 0 || !fused_pass_supported(dtype, g, k)) continue;
+    const int64_t v = fused_pass_visits(dtype, g, k);
+    if (v > 0) c.us[k] = tab[k] * (float)v / (float)ref[k];
   }
   return c;
 }

@@ -387,15 +387,28 @@ def test_auto_plan_scales_launch_times_to_larger_grids():
     pass's length on the grid (rounds of resident blocks x planes marched
     per block): at 1024^3 FP64 runs mostly 8-step passes (679 us per step
     measured on MI355X against 710 for 7-step ones), at 512^3 7-step passes
-    (86 us per step against 91).  Smaller grids keep the 512^3 times."""
+    (86 us per step against 91)."""
     L = _lib()
     big = ((1024, 1024, 1024), (0.7, -0.4, 0.2), (1.0, 1.0, 1.0))
     ks = _passes(L.F64, 100, 0, big)
     assert sum(ks) == 100 and ks.count(8) >= 9, ks
     ks = _passes(L.F64, 100, 0)
     assert ks.count(7) >= 10, ks
-    small = ((21, 70, 130), (0.3, -0.45, 0.2), (1.0, 1.0, 1.0))
-    assert sorted(_passes(L.F64, 20, 0, small), reverse=True) == [7, 7, 6]
+
+
+def test_auto_plan_on_small_grids_follows_measured_times():
+    """Below 512^3 the planner interpolates launch times measured at 16^3,
+    128^3 and 256^3 (latency-bound marches of few planes): 4-step passes at
+    16^3 and 21x70x130 (FP64 2.5 us per step against 3.3 for 7-step ones),
+    FP32 4-step passes at 128^3 (4.1 us per step against 5.6 for 8-step
+    ones), bf16 with stochastic rounding one step per launch there (5.9 us
+    against 6.2 for 4-step passes)."""
+    L = _lib()
+    v, lens = (0.3, -0.45, 0.2), (1.0, 1.0, 1.0)
+    assert _passes(L.F64, 20, 0, ((16, 16, 16), v, lens)) == [4] * 5
+    assert _passes(L.F64, 20, 0, ((21, 70, 130), v, lens)) == [4] * 5
+    assert _passes(L.F32, 100, 0, ((128, 128, 128), v, lens)) == [4] * 25
+    assert _passes(L.BF16_SR, 10, 0, ((128, 128, 128), v, lens)) == [1] * 10
 
 
 @pytest.mark.parametrize("dtype", ["F64", "F32", "BF16", "BF16_SR"])
