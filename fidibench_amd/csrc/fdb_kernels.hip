@@ -1038,7 +1038,7 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
 #pragma unroll
       for (int r = 0; r < R; ++r) nx[r] = St<TS>::ld(buf_ld(row_rsrc(base + roff[r], rbytes), cb, in));
     }
-    double ps[3] = {0.0, 0.0, 0.0};  // PROD: this plane's products, all levels
+    double ps[3] = {0.0, 0.0, 0.0};  // PROD: this plane's products (LASTP: last level)
 #pragma unroll
     for (int l = 0; l < K; ++l, ++lvl) {
       // Level l+1 from level l in three phases over all rows -- the three

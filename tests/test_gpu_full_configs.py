@@ -46,10 +46,15 @@ def _compare(a, b):
 
 
 def test_mixed_precision_gauss1024():
+    """Every storage type in the planner's split (time_block 0: at 1024^3
+    the 512^3 launch times scaled by each pass's length), which must equal
+    one step per launch (FP64) and 8 steps per pass (bf16 stochastic
+    rounding, whose split is 4-step passes) bit for bit."""
     from fidibench_amd import Upwind, configs as K
     w = K.gauss1024()
     ups, drift = {}, {}
-    for name, tb in (("f64", 8), ("f64_k1", 1), ("f32", 8), ("bf16", 8), ("bf16sr", 8)):
+    for name, tb in (("f64", 0), ("f64_k1", 1), ("f32", 0), ("bf16", 0), ("bf16sr", 0),
+                     ("bf16sr_k8", 8)):
         up = Upwind(w.velocity, w.lengths, w.num_cells, dtype=name.split("_")[0])
         up.set_time_block(tb)
         up.init_field(w.init_kind, w.init_params, seed=w.init_seed)
@@ -59,6 +64,9 @@ def test_mixed_precision_gauss1024():
         ups[name] = up
     same = _compare(ups["f64_k1"], ups["f64"])
     assert same["max_abs"] == 0.0 and ups["f64_k1"].mass() == ups["f64"].mass()
+    sr8 = ups.pop("bf16sr_k8")
+    assert _compare(sr8, ups["bf16sr"])["max_abs"] == 0.0 and sr8.mass() == ups["bf16sr"].mass()
+    sr8.close()
     assert drift["f64"] < 1e-13, drift
     errs = {n: _compare(ups[n], ups["f64"]) for n in ("f32", "bf16", "bf16sr")}
     print("gauss1024 errors vs FP64", errs, "drift", drift)
