@@ -234,7 +234,8 @@ def test_shift_method_equals_checkpointed_sweep(dtype, shape, v, T):
 @pytest.mark.parametrize("pair", ["f64/f64", "f32/f32", "bf16/f32"])
 @pytest.mark.parametrize("shape,v,T", [((16, 66, 130), (0.45, -0.3, 0.2), 7),
                                        ((21, 50, 67), (-0.6, 0.25, -0.3), 8),
-                                       ((9, 130, 70), (0.3, 0.4, -0.2), 23)])
+                                       ((9, 130, 70), (0.3, 0.4, -0.2), 23),
+                                       ((5, 9, 17), (-0.45, 0.3, -0.2), 9)])
 def test_shift_last_pass_products_equal_one_step_products(pair, shape, v, T):
     """The shift-invariant backward ends with one fused pass that forms the
     velocity products at its last level (7 steps for FP64 lam, 8 for FP32
