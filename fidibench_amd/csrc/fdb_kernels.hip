@@ -1147,32 +1147,35 @@ __global__ __launch_bounds__(64 * WY) void tb_kernel(const TS* __restrict__ in,
         if (LO1 ? w > 0 : w + 1 < WY) n1 = e;
       }
       d1[er] = n1 - x[er];
-      if constexpr (PROD) if (!LASTP || l + 1 == K) {
-        const int fl = LASTP ? 0 : l;
-        const bool cells = m >= K && col_out;
+      if constexpr (PROD) {
+        if (!LASTP || l + 1 == K) {
+          const int fl = LASTP ? 0 : l;
+          const bool cells = m >= K && col_out;
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const double fd = (double)fv[fl][r];
-          double q0, q1, q2;
-          if constexpr (sizeof(M) == 8) {  // the FP64 differences are d0..d2
-            q0 = fd * d0[r];
-            q1 = fd * d1[r];
-            q2 = fd * d2[r];
-          } else {
-            const double xd = (double)x[r];
-            const M v1 = r == er ? n1 : x[LO1 ? (r > 0 ? r - 1 : 0) : (r + 1 < R ? r + 1 : R - 1)];
-            q0 = fd * ((double)prev[l][r] - xd);
-            q1 = fd * ((double)v1 - xd);
-            q2 = fd * ((double)nb2[r] - xd);
+          for (int r = 0; r < R; ++r) {
+            const double fd = (double)fv[fl][r];
+            double q0, q1, q2;
+            if constexpr (sizeof(M) == 8) {  // the FP64 differences are d0..d2
+              q0 = fd * d0[r];
+              q1 = fd * d1[r];
+              q2 = fd * d2[r];
+            } else {
+              const double xd = (double)x[r];
+              const M v1 =
+                  r == er ? n1 : x[LO1 ? (r > 0 ? r - 1 : 0) : (r + 1 < R ? r + 1 : R - 1)];
+              q0 = fd * ((double)prev[l][r] - xd);
+              q1 = fd * ((double)v1 - xd);
+              q2 = fd * ((double)nb2[r] - xd);
+            }
+            const bool ok = cells && row_out[r];
+            ps[0] += ok ? q0 : 0.0;
+            ps[1] += ok ? q1 : 0.0;
+            ps[2] += ok ? q2 : 0.0;
           }
-          const bool ok = cells && row_out[r];
-          ps[0] += ok ? q0 : 0.0;
-          ps[1] += ok ? q1 : 0.0;
-          ps[2] += ok ? q2 : 0.0;
-        }
-        const TF* fb = (const TF*)adj.fs[fl] + pn * plane;
+          const TF* fb = (const TF*)adj.fs[fl] + pn * plane;
 #pragma unroll
-        for (int r = 0; r < R; ++r) fv[fl][r] = (M)St<TF>::ld((fb + rofff[r])[colf]);
+          for (int r = 0; r < R; ++r) fv[fl][r] = (M)St<TF>::ld((fb + rofff[r])[colf]);
+        }
       }
 #pragma unroll
       for (int r = 0; r < R; ++r) {
