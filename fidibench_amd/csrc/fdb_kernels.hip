@@ -1352,7 +1352,7 @@ constexpr int kRowsShapes[][2] = {{8, 2}, {4, 2}, {4, 4}, {8, 1}, {2, 4}, {2, 8}
 constexpr int kTbShapes[][3] = {{9, 2, 4}, {9, 3, 4}, {9, 4, 4}, {9, 6, 8},
                                 {9, 7, 8}, {9, 8, 8}, {5, 8, 8}};
 // FP64-only shapes: two planes per trip fit in 256 VGPRs.
-constexpr int kTbShapes64[][3] = {{8, 7, 8}, {7, 8, 8}};
+constexpr int kTbShapes64[][3] = {{8, 6, 8}, {8, 7, 8}, {7, 8, 8}};
 
 // Default launch shape of the rows kernel.  bytes: storage of the stepped
 // field (lam for the adjoint); fbytes: the adjoint's f_t storage (0: none).
@@ -1404,6 +1404,7 @@ TbCfg tb_default(int bytes, int K) {
 struct Tuning {
   RowsCfg rows[2][9][9];  // [adj][bytes][fbytes]
   TbCfg tb[9][kMaxTimeBlock + 1];  // [bytes][K]
+  bool tb_env[9][kMaxTimeBlock + 1] = {};  // set by FDB_TB_CFG_<bytes>_<K>
   std::string error;
 };
 
@@ -1490,6 +1491,7 @@ Tuning load_tuning() {
           c = TbCfg{v[0], v[1], v[2]};
         else
           bad(name, s);
+        t.tb_env[b][K] = true;
       }
       t.tb[b][K] = c;
     }
@@ -1623,10 +1625,30 @@ bool tb_sr_ok(const Grid3& g, int K) {
   return (K == 2 || K == 4 || K == 8) && g.n[1] % 2 == 0;
 }
 
-// visits != nullptr: launch nothing; store the pass's length in plane visits
-// of one resident block slot instead (rounds of resident blocks x planes per
-// block, warm-up planes included), from the same shape, chunking and
-// occupancy the launch would use.
+// FP64 at K = 6: 9 rows x 8 waves (at 512^3 72 tiles, two rounds of 504
+// blocks) or 8 rows (81 tiles, one round of 243), whichever marches fewer
+// row-planes per resident block slot on this grid: the march's rounds x
+// (chunk + K) planes, times the rows per wave.  Measured: 512^3 picks 8
+// rows, 549 against 570 us per launch; 1024^3 9 rows, 4751 against 4844.
+int fp64_k6_rows(const Grid3& g) {
+  constexpr int K = 6, WY = 8;
+  auto cost = [&](int R, int64_t cap) {
+    const int64_t tiles =
+        ((g.n[2] + (64 - K) - 1) / (64 - K)) * ((g.n[1] + (WY * R - K) - 1) / (WY * R - K));
+    const int64_t nch = march_chunks(tiles, g.n[0], K, cap, 1, 64, true);
+    const int64_t chunk = (g.n[0] + nch - 1) / nch;
+    const int64_t blocks = tiles * ((g.n[0] + chunk - 1) / chunk);
+    return (blocks + cap - 1) / cap * (chunk + K) * R;
+  };
+  const int64_t c9 = cost(9, resident_blocks(tb_kernel<double, 9, K, WY, false, false>, 64 * WY));
+  const int64_t c8 = cost(8, resident_blocks(tb_kernel<double, 8, K, WY, false, false>, 64 * WY));
+  return c8 < c9 ? 8 : 9;
+}
+
+// visits != nullptr: launch nothing; store the pass's length in row-plane
+// visits of one resident block slot instead (rounds of resident blocks x
+// planes per block, warm-up planes included, x rows per wave), from the same
+// shape, chunking and occupancy the launch would use.
 template <typename TS>
 hipError_t tb_dispatch(const Grid3& g, const TS* in, TS* out, int K, hipStream_t s,
                        void* const* levels = nullptr, const uint32_t* sr_keys = nullptr,
@@ -1634,7 +1656,10 @@ hipError_t tb_dispatch(const Grid3& g, const TS* in, TS* out, int K, hipStream_t
   using M = typename St<TS>::M;
   const bool sr = sr_keys != nullptr;
   if (sr && (sizeof(TS) != 2 || !tb_sr_ok(g, K))) return hipErrorNotSupported;
-  const TbCfg cfg = sr ? tb_sr_cfg(K) : tb_cfg((int)sizeof(TS), K);
+  TbCfg cfg = sr ? tb_sr_cfg(K) : tb_cfg((int)sizeof(TS), K);
+  if constexpr (sizeof(TS) == 8) {
+    if (K == 6 && !sr && levels == nullptr && !tuning().tb_env[8][6]) cfg.R = fp64_k6_rows(g);
+  }
   const int R = cfg.R, WY = cfg.WY;
   if (!(g.act[0] && g.act[1] && g.act[2]) || WY * R <= K) return hipErrorNotSupported;
   TbArgs a;
@@ -1672,7 +1697,7 @@ hipError_t tb_dispatch(const Grid3& g, const TS* in, TS* out, int K, hipStream_t
     a.chunk = (a.n0 + nch - 1) / nch;                                                       \
     a.ntiles = tiles * ((a.n0 + a.chunk - 1) / a.chunk);                                    \
     if (visits) {                                                                           \
-      *visits = (a.ntiles + cap - 1) / cap * (a.chunk + KK);                                \
+      *visits = (a.ntiles + cap - 1) / cap * (a.chunk + KK) * RR;                           \
     } else {                                                                                \
       if (a.ntiles > 0x7fffffffLL) return hipErrorInvalidConfiguration;                     \
       hipLaunchKernelGGL(kern, dim3((unsigned)a.ntiles), dim3(64, W), 0, s, in, out, a, cf, \
@@ -1691,6 +1716,7 @@ hipError_t tb_dispatch(const Grid3& g, const TS* in, TS* out, int K, hipStream_t
   FDB_TB(9, 8, 8, kTbPlain)
   FDB_TB(5, 8, 8, kTbPlain)
   if constexpr (sizeof(TS) == 8) {
+    FDB_TB(8, 6, 8, kTbPlain)
     FDB_TB(8, 7, 8, kTbPlain)
     FDB_TB(7, 8, 8, kTbPlain)
     FDB_TB(9, 4, 4, kTbStoreAll)
