@@ -1900,7 +1900,7 @@ constexpr float kPassUsAt[kPassSizes][4][kMaxTimeBlock + 1] = {
      {0, 30.4f, 39.2f, 41.5f, 43.4f, 0, 57.0f, 70.8f, 80.2f},
      {0, 17.7f, 28.9f, 35.8f, 44.6f, 0, 72.5f, 78.7f, 91.7f},
      {0, 20.7f, 33.9f, 0, 67.7f, 0, 0, 0, 144.0f}},
-    {{0, 373, 573, 576, 537, 0, 553, 577, 705},
+    {{0, 373, 573, 576, 537, 0, 549, 577, 705},
      {0, 190, 379, 407, 381, 0, 400, 450, 402},
      {0, 99, 228, 233, 241, 0, 394, 356, 453},
      {0, 117, 220, 0, 348, 0, 0, 0, 769}}};
